@@ -1,0 +1,537 @@
+// C ABI of libdgk (include/dgk.h): context setup, state transfer and time stepping.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <unordered_map>
+
+#include "dgk_internal.hpp"
+#include "refelem.hpp"
+
+namespace dgk {
+
+Ctx::~Ctx() {
+  if (graph) (void)hipGraphExecDestroy(graph);
+  for (auto& s : sources)
+    if (s.idx) (void)hipFree(s.idx);
+  for (void* p : allocs) (void)hipFree(p);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
+
+thread_local std::string g_err;
+
+dgk_status fail(Ctx* c, dgk_status st, const std::string& msg) {
+  g_err = msg;
+  if (c) c->err = msg;
+  return st;
+}
+
+template <class F>
+dgk_status guarded(Ctx* c, F&& f) {
+  try {
+    f();
+    return DGK_OK;
+  } catch (InvalidArg& e) {
+    return fail(c, DGK_ERR_INVALID, e.what());
+  } catch (Unsupported& e) {
+    return fail(c, DGK_ERR_UNSUPPORTED, e.what());
+  } catch (MeshError& e) {
+    return fail(c, DGK_ERR_MESH, e.what());
+  } catch (DeviceError& e) {
+    return fail(c, DGK_ERR_DEVICE, e.what());
+  } catch (std::bad_alloc&) {
+    return fail(c, DGK_ERR_DEVICE, "host allocation failed");
+  } catch (std::exception& e) {
+    return fail(c, DGK_ERR_INVALID, e.what());
+  }
+}
+
+// derivative of the equispaced order-p vertex Lagrange function psi(l) = prod_k (p l - k)/(k+1)
+double dpsi(int p, double l) {
+  double s = 0;
+  for (int m = 0; m < p; ++m) {
+    double t = (double)p / (m + 1);
+    for (int k = 0; k < p; ++k)
+      if (k != m) t *= (p * l - k) / (k + 1);
+    s += t;
+  }
+  return s;
+}
+
+// Sign of the reference normal at the first face quadrature point relative to the
+// right-hand normal of the face's first vertices: grad(phi_0) x grad(phi_1) (3D) or
+// grad(phi_0) x (0,0,-1) (2D) with phi_v = psi(lambda_v) (src/Mesh.cpp:173-188).
+double normal_sigma(int dim, int p) {
+  if (dim == 1) return 1;
+  std::vector<ld> x, w;
+  gauss_legendre(p + 2, x, w);
+  double xi0 = (double)x[0];
+  if (dim == 2) {
+    double l0 = (1 - xi0) / 2;
+    return dpsi(p, l0) >= 0 ? 1 : -1;
+  }
+  double t0 = (xi0 + 1) / 2;
+  double u = t0, v = (1 - t0) * t0;
+  double l0 = 1 - u - v, l1 = u;
+  return dpsi(p, l0) * dpsi(p, l1) >= 0 ? 1 : -1;
+}
+
+template <typename T>
+void upload_ops(Ctx& c, const RefElem& R, const std::vector<int>& vmap) {
+  std::vector<T> W(R.W.begin(), R.W.end()), L(R.LIFT.begin(), R.LIFT.end());
+  c.ops.W = c.alloc(W.size() * sizeof(T));
+  c.ops.LIFT = c.alloc(L.size() * sizeof(T));
+  DGK_HIP(hipMemcpy(c.ops.W, W.data(), W.size() * sizeof(T), hipMemcpyHostToDevice));
+  DGK_HIP(hipMemcpy(c.ops.LIFT, L.data(), L.size() * sizeof(T), hipMemcpyHostToDevice));
+  std::vector<int> fm;
+  for (auto& f : R.fmask) fm.insert(fm.end(), f.begin(), f.end());
+  c.ops.fmask = (int*)c.alloc(fm.size() * sizeof(int));
+  DGK_HIP(hipMemcpy(c.ops.fmask, fm.data(), fm.size() * sizeof(int), hipMemcpyHostToDevice));
+  c.ops.vmap = (int*)c.alloc(vmap.size() * sizeof(int));
+  DGK_HIP(hipMemcpy(c.ops.vmap, vmap.data(), vmap.size() * sizeof(int), hipMemcpyHostToDevice));
+}
+
+// vmap[(jn*nperm + perm)*Nfp + k]: neighbour element node matching own face node k when the
+// own face's vertex slot m is the neighbour face's slot pi_perm[m].
+std::vector<int> make_vmap(const RefElem& R, int nperm) {
+  const int d = R.dim, p = R.p;
+  std::vector<std::array<int, 3>> flat;
+  if (d == 1) flat = {{0, 0, 0}};
+  if (d == 2) for (auto a : line_lattice(p)) flat.push_back({a[0], a[1], 0});
+  if (d == 3) flat = tri_lattice(p);
+  static const int P6[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  static const int P2[2][3] = {{0, 1, 2}, {1, 0, 2}};
+  std::vector<int> vm(R.Nf * nperm * R.Nfp, 0);
+  for (int jn = 0; jn < R.Nf; ++jn)
+    for (int pm = 0; pm < nperm; ++pm)
+      for (int k = 0; k < R.Nfp; ++k) {
+        int kk = 0;
+        if (d > 1) {
+          const int* pi = d == 3 ? P6[pm] : P2[pm];
+          std::array<int, 3> b{0, 0, 0};
+          for (int m = 0; m < d; ++m) b[pi[m]] = flat[k][m];
+          kk = -1;
+          for (int q = 0; q < R.Nfp; ++q)
+            if (flat[q] == b) kk = q;
+          if (kk < 0) throw std::runtime_error("face lattice permutation not found");
+        }
+        vm[(jn * nperm + pm) * R.Nfp + k] = R.fmask[jn][kk];
+      }
+  return vm;
+}
+
+void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* o) {
+  if (!m || !ph) throw InvalidArg("mesh and phys must not be NULL");
+  if (m->dim < 1 || m->dim > 3) throw Unsupported("element dimension must be 1, 2 or 3");
+  if (m->order < 1 || m->order > 5) throw Unsupported("element order must be 1..5");
+  if (m->num_elements <= 0 || !m->node_xyz || !m->node_tags) throw InvalidArg("empty mesh");
+  if (!(ph->c0 > 0) || !(ph->rho0 > 0)) throw InvalidArg("c0 and rho0 must be positive");
+  c.dim = m->dim;
+  c.p = m->order;
+  c.K = m->num_elements;
+  c.precision = o ? o->precision : DGK_FP64;
+  c.sign_mode = o ? o->flux_sign : DGK_SIGN_REFERENCE;
+  c.device = o ? o->device : 0;
+  c.variant = o ? o->variant : 0;
+  if (c.precision != DGK_FP64 && c.precision != DGK_FP32) throw InvalidArg("precision must be DGK_FP64 or DGK_FP32");
+  if (c.sign_mode != DGK_SIGN_REFERENCE && c.sign_mode != DGK_SIGN_STABLE) throw InvalidArg("bad flux_sign");
+  c.tsize = c.precision == DGK_FP64 ? 8 : 4;
+  c.c0 = ph->c0;
+  c.rho0 = ph->rho0;
+  for (int k = 0; k < 3; ++k) c.v0[k] = ph->v0[k];
+  c.fc = (c.dim == 3 && c.p != 1) ? -1 : 1;  // src/Mesh.cpp:210-211
+  RefElem R = make_ref_elem(c.dim, c.p);
+  c.Np = R.Np;
+  c.Nf = R.Nf;
+  c.Nfp = R.Nfp;
+  c.nperm = c.dim == 3 ? 6 : c.dim == 2 ? 2 : 1;
+  for (int j = 0; j < c.Nf; ++j) {
+    for (int x = 0; x < 3; ++x) c.refn[j][x] = R.ref_normals[j][x];
+    c.fsc[j] = R.ref_face_measure[j] / R.face_ref_measure;
+  }
+  const int D = c.dim, Np = c.Np, NV = D + 1;
+  const int64_t K = c.K;
+  // affine (straight-sided) check and vertex arrays
+  std::vector<double> vxyz(K * NV * 3);
+  for (int64_t e = 0; e < K; ++e) {
+    const double* X = m->node_xyz + e * Np * 3;
+    for (int v = 0; v < NV; ++v)
+      for (int x = 0; x < 3; ++x) vxyz[(e * NV + v) * 3 + x] = X[v * 3 + x];
+    double diam = 0;
+    for (int v = 1; v < NV; ++v)
+      for (int x = 0; x < 3; ++x) diam = std::max(diam, std::fabs(X[v * 3 + x] - X[x]));
+    for (int n = NV; n < Np; ++n)
+      for (int x = 0; x < 3; ++x) {
+        double s = 0;
+        for (int v = 0; v < NV; ++v) s += (double)R.lattice[n][v] * X[v * 3 + x];
+        s /= c.p;
+        if (std::fabs(s - X[n * 3 + x]) > 1e-9 * diam)
+          throw Unsupported("element " + std::to_string(e) +
+                            " is curved or its high-order nodes are not in Gmsh order; only straight-sided "
+                            "elements are supported");
+      }
+  }
+  // dense corner-vertex ids from node tags
+  std::vector<int32_t> vid(K * NV);
+  {
+    std::unordered_map<int64_t, int32_t> ids;
+    ids.reserve(K * 2);
+    for (int64_t e = 0; e < K; ++e)
+      for (int v = 0; v < NV; ++v) {
+        int64_t t = m->node_tags[e * Np + v];
+        auto it = ids.find(t);
+        int32_t id;
+        if (it == ids.end()) { id = (int32_t)ids.size(); ids.emplace(t, id); }
+        else id = it->second;
+        vid[e * NV + v] = id;
+      }
+    if (D == 3 && ids.size() >= ((size_t)1 << 21)) throw Unsupported("more than 2^21 mesh vertices");
+  }
+  std::vector<uint8_t> bcf(K * c.Nf, 0);
+  if (m->node_bc)
+    for (int64_t e = 0; e < K; ++e)
+      for (int j = 0; j < c.Nf; ++j) bcf[e * c.Nf + j] = m->node_bc[e * Np + R.fmask[j][0]] ? 1 : 0;
+
+  DGK_HIP(hipSetDevice(c.device));
+  DGK_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  const int64_t n = 4 * K * Np;
+  c.U = c.alloc(n * c.tsize);
+  c.A = c.alloc(n * c.tsize);
+  c.V0 = c.alloc(n * c.tsize);
+  c.V1 = c.alloc(n * c.tsize);
+  c.Jinv = c.alloc(K * 9 * c.tsize);
+  c.nbr = (int32_t*)c.alloc(K * c.Nf * sizeof(int32_t));
+  c.code = (int32_t*)c.alloc(K * c.Nf * sizeof(int32_t));
+  c.nface = (int32_t*)c.alloc(K * c.Nf * sizeof(int32_t));
+  c.perm = (int32_t*)c.alloc(K * c.Nf * sizeof(int32_t));
+  c.flag = (int*)c.alloc(sizeof(int));
+  DGK_HIP(hipMemsetAsync(c.U, 0, n * c.tsize, c.stream));
+  DGK_HIP(hipMemsetAsync(c.A, 0, n * c.tsize, c.stream));
+  double* vx = nullptr;
+  DGK_HIP(hipMalloc(&vx, vxyz.size() * sizeof(double)));
+  try {
+    DGK_HIP(hipMemcpyAsync(vx, vxyz.data(), vxyz.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    build_connectivity(c, vx, vid.data(), bcf.data(), normal_sigma(D, c.p));
+    build_geometry(c, vx);
+  } catch (...) {
+    (void)hipFree(vx);
+    throw;
+  }
+  DGK_HIP(hipFree(vx));
+  auto vmap = make_vmap(R, c.nperm);
+  if (c.precision == DGK_FP64) upload_ops<double>(c, R, vmap);
+  else upload_ops<float>(c, R, vmap);
+  // conformity: the device table must pair face nodes carrying equal tags (src/Mesh.cpp:253-263)
+  std::vector<int32_t> nb(K * c.Nf), cd(K * c.Nf);
+  DGK_HIP(hipMemcpy(nb.data(), c.nbr, nb.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  DGK_HIP(hipMemcpy(cd.data(), c.code, cd.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  int64_t nbnd = 0;
+  for (int64_t e = 0; e < K; ++e)
+    for (int j = 0; j < c.Nf; ++j) {
+      int32_t o = nb[e * c.Nf + j];
+      if (o < 0) { ++nbnd; continue; }
+      int sel = cd[e * c.Nf + j] & 0xffff;
+      for (int k = 0; k < c.Nfp; ++k)
+        if (m->node_tags[e * Np + R.fmask[j][k]] != m->node_tags[(int64_t)o * Np + vmap[sel * c.Nfp + k]])
+          throw MeshError("non-conforming face between elements " + std::to_string(e) + " and " + std::to_string(o));
+    }
+  c.Fb = nbnd;
+  c.F = (K * c.Nf + nbnd) / 2;
+}
+
+StageArgs base_args(Ctx& c) {
+  StageArgs a{};
+  a.K = c.K;
+  a.U = c.U;
+  a.A = c.A;
+  a.Jinv = c.Jinv;
+  a.nbr = c.nbr;
+  a.code = c.code;
+  a.W = c.ops.W;
+  a.LIFT = c.ops.LIFT;
+  a.fmask = c.ops.fmask;
+  a.vmap = c.ops.vmap;
+  for (int j = 0; j < kMaxNF; ++j) {
+    for (int x = 0; x < 3; ++x) a.refn[j][x] = c.refn[j][x];
+    a.fsc[j] = c.fsc[j];
+  }
+  a.c0 = c.c0;
+  a.rho0 = c.rho0;
+  for (int k = 0; k < 3; ++k) a.v0[k] = c.v0[k];
+  return a;
+}
+
+// Launch the four RK4 stages of one step on stream s.
+void rk4_step(Ctx& c, double dt, hipStream_t s, hipEvent_t* ev = nullptr) {
+  StageArgs a = base_args(c);
+  a.dt = dt;
+  void* in[4] = {c.U, c.V0, c.V1, c.V0};
+  void* out[4] = {c.V0, c.V1, c.V0, nullptr};
+  for (int st = 0; st < 4; ++st) {
+    a.stage = st;
+    a.Vin = in[st];
+    a.Vout = out[st];
+    if (ev) DGK_HIP(hipEventRecord(ev[2 * st], s));
+    launch_stage(c, a, s);
+    if (ev) DGK_HIP(hipEventRecord(ev[2 * st + 1], s));
+  }
+}
+
+void apply_sources(Ctx& c, double t) {
+  for (auto& src : c.sources)
+    if (t < src.duration) launch_source(c, src, src.amp * std::sin(2 * M_PI * src.freq * t + src.phase), c.stream);
+}
+
+bool sources_active(Ctx& c, double t) {
+  for (auto& src : c.sources)
+    if (t < src.duration && src.n) return true;
+  return false;
+}
+
+void rk4_impl(Ctx& c, int nsteps, double dt, const double* ts) {
+  if (nsteps < 0) throw InvalidArg("nsteps < 0");
+  constexpr int kChunk = 25;
+  int i = 0;
+  while (i < nsteps) {
+    double t = ts ? ts[i] : 0.0;
+    bool src = !c.sources.empty() && ts && sources_active(c, t);
+    if (!src && nsteps - i >= kChunk) {
+      // graph of kChunk steps (no sources active in it)
+      bool any = false;
+      for (int k = i; k < i + kChunk && ts && !c.sources.empty(); ++k) any |= sources_active(c, ts[k]);
+      if (!any) {
+        if (!c.graph || c.graph_steps != kChunk || c.graph_dt != dt) {
+          if (c.graph) { DGK_HIP(hipGraphExecDestroy(c.graph)); c.graph = nullptr; }
+          hipGraph_t g;
+          DGK_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+          try {
+            for (int k = 0; k < kChunk; ++k) rk4_step(c, dt, c.stream);
+          } catch (...) {
+            hipGraph_t junk;
+            (void)hipStreamEndCapture(c.stream, &junk);
+            throw;
+          }
+          DGK_HIP(hipStreamEndCapture(c.stream, &g));
+          hipError_t e = hipGraphInstantiate(&c.graph, g, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(g);
+          DGK_HIP(e);
+          c.graph_steps = kChunk;
+          c.graph_dt = dt;
+        }
+        DGK_HIP(hipGraphLaunch(c.graph, c.stream));
+        i += kChunk;
+        continue;
+      }
+    }
+    if (ts) apply_sources(c, t);
+    rk4_step(c, dt, c.stream);
+    ++i;
+  }
+}
+
+void euler_impl(Ctx& c, int nsteps, double dt, const double* ts) {
+  for (int i = 0; i < nsteps; ++i) {
+    if (ts) apply_sources(c, ts[i]);
+    StageArgs a = base_args(c);
+    a.dt = dt;
+    a.stage = 5;
+    a.Vin = c.U;
+    a.Vout = c.V0;
+    launch_stage(c, a, c.stream);
+    std::swap(c.U, c.V0);  // graphs capture buffer addresses; drop any cached one
+    if (c.graph) { DGK_HIP(hipGraphExecDestroy(c.graph)); c.graph = nullptr; }
+  }
+}
+
+double* staging(Ctx& c) {
+  if (!c.stage_d) c.stage_d = (double*)c.alloc(4 * c.K * c.Np * sizeof(double));
+  return c.stage_d;
+}
+
+}  // namespace
+}  // namespace dgk
+
+using namespace dgk;
+
+extern "C" {
+
+dgk_status dgk_create(const dgk_mesh* mesh, const dgk_phys* phys, const dgk_opts* opts, dgk_ctx** out) {
+  if (!out) return fail(nullptr, DGK_ERR_INVALID, "out must not be NULL");
+  *out = nullptr;
+  dgk_ctx* c = new dgk_ctx();
+  dgk_status st = guarded(c, [&] { create_impl(*c, mesh, phys, opts); });
+  if (st != DGK_OK) {
+    g_err = c->err;
+    delete c;
+    return st;
+  }
+  *out = c;
+  return DGK_OK;
+}
+
+dgk_status dgk_destroy(dgk_ctx* c) {
+  if (!c) return DGK_OK;
+  delete c;
+  return DGK_OK;
+}
+
+const char* dgk_last_error(const dgk_ctx* c) { return c ? c->err.c_str() : g_err.c_str(); }
+
+dgk_status dgk_info_get(const dgk_ctx* c, dgk_info* info) {
+  if (!c || !info) return fail(nullptr, DGK_ERR_INVALID, "NULL argument");
+  info->dim = c->dim; info->order = c->p; info->Np = c->Np; info->Nf = c->Nf; info->Nfp = c->Nfp;
+  info->num_elements = c->K; info->num_faces = c->F; info->num_boundary_faces = c->Fb;
+  info->precision = c->precision; info->variant = c->variant; info->fc = c->fc;
+  info->device_bytes = c->device_bytes;
+  return DGK_OK;
+}
+
+dgk_status dgk_set_state(dgk_ctx* c, const double* u) {
+  if (!c || !u) return fail(c, DGK_ERR_INVALID, "NULL argument");
+  return guarded(c, [&] {
+    int64_t n = 4 * c->K * c->Np;
+    if (c->precision == DGK_FP64) {
+      DGK_HIP(hipMemcpyAsync(c->U, u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    } else {
+      double* s = staging(*c);
+      DGK_HIP(hipMemcpyAsync(s, u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      convert_state(*c, s, c->U, n);
+    }
+    DGK_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+dgk_status dgk_get_state(dgk_ctx* c, double* u) {
+  if (!c || !u) return fail(c, DGK_ERR_INVALID, "NULL argument");
+  return guarded(c, [&] {
+    int64_t n = 4 * c->K * c->Np;
+    if (c->precision == DGK_FP64) {
+      DGK_HIP(hipMemcpyAsync(u, c->U, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    } else {
+      double* s = staging(*c);
+      convert_state_back(*c, c->U, s, n);
+      DGK_HIP(hipMemcpyAsync(u, s, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    DGK_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+dgk_status dgk_set_sources(dgk_ctx* c, int n, const double* src, const double* node_xyz) {
+  if (!c || n < 0 || (n > 0 && (!src || !node_xyz))) return fail(c, DGK_ERR_INVALID, "bad source arguments");
+  return guarded(c, [&] {
+    DGK_HIP(hipStreamSynchronize(c->stream));
+    for (auto& s : c->sources)
+      if (s.idx) DGK_HIP(hipFree(s.idx));
+    c->sources.clear();
+    const int64_t KN = c->K * c->Np;
+    for (int s = 0; s < n; ++s) {
+      const double* S = src + 9 * s;
+      std::vector<int> idx;
+      for (int64_t i = 0; i < KN; ++i) {
+        const double* X = node_xyz + 3 * i;
+        if (std::pow(X[0] - S[1], 2) + std::pow(X[1] - S[2], 2) + std::pow(X[2] - S[3], 2) < std::pow(S[4], 2))
+          idx.push_back((int)i);  // pressure is equation 0: U[0*KN + i]
+      }
+      Ctx::Src d;
+      d.amp = S[5]; d.freq = S[6]; d.phase = S[7]; d.duration = S[8];
+      d.n = (int64_t)idx.size();
+      if (d.n) {
+        DGK_HIP(hipMalloc(&d.idx, idx.size() * sizeof(int)));
+        DGK_HIP(hipMemcpy(d.idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+      }
+      c->sources.push_back(d);
+    }
+  });
+}
+
+dgk_status dgk_rk4(dgk_ctx* c, int nsteps, double dt, const double* t_start) {
+  if (!c) return fail(nullptr, DGK_ERR_INVALID, "NULL context");
+  return guarded(c, [&] { rk4_impl(*c, nsteps, dt, t_start); });
+}
+
+dgk_status dgk_euler(dgk_ctx* c, int nsteps, double dt, const double* t_start) {
+  if (!c) return fail(nullptr, DGK_ERR_INVALID, "NULL context");
+  return guarded(c, [&] { euler_impl(*c, nsteps, dt, t_start); });
+}
+
+dgk_status dgk_rhs(dgk_ctx* c, const double* u, double* out, double dt) {
+  if (!c || !u || !out) return fail(c, DGK_ERR_INVALID, "NULL argument");
+  return guarded(c, [&] {
+    int64_t n = 4 * c->K * c->Np;
+    double* s = staging(*c);
+    DGK_HIP(hipMemcpyAsync(s, u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    convert_state(*c, s, c->V1, n);
+    StageArgs a = base_args(*c);
+    a.dt = dt;
+    a.stage = 4;
+    a.Vin = c->V1;
+    a.Vout = c->V0;
+    launch_stage(*c, a, c->stream);
+    convert_state_back(*c, c->V0, s, n);
+    DGK_HIP(hipMemcpyAsync(out, s, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    DGK_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+dgk_status dgk_sync(dgk_ctx* c) {
+  if (!c) return fail(nullptr, DGK_ERR_INVALID, "NULL context");
+  return guarded(c, [&] { DGK_HIP(hipStreamSynchronize(c->stream)); });
+}
+
+dgk_status dgk_check_finite(dgk_ctx* c) {
+  if (!c) return fail(nullptr, DGK_ERR_INVALID, "NULL context");
+  bool ok = true;
+  dgk_status st = guarded(c, [&] { ok = check_finite(*c); });
+  if (st != DGK_OK) return st;
+  return ok ? DGK_OK : fail(c, DGK_ERR_NONFINITE, "state contains NaN or Inf");
+}
+
+dgk_status dgk_connectivity(dgk_ctx* c, int32_t* nbr, int32_t* nface, int32_t* perm, int32_t* sign) {
+  if (!c) return fail(nullptr, DGK_ERR_INVALID, "NULL context");
+  return guarded(c, [&] {
+    size_t n = c->K * c->Nf;
+    std::vector<int32_t> cd(n);
+    if (nbr) DGK_HIP(hipMemcpy(nbr, c->nbr, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (nface) DGK_HIP(hipMemcpy(nface, c->nface, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (perm) DGK_HIP(hipMemcpy(perm, c->perm, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (sign) {
+      std::vector<int32_t> nb(n);
+      DGK_HIP(hipMemcpy(cd.data(), c->code, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+      DGK_HIP(hipMemcpy(nb.data(), c->nbr, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < n; ++i) sign[i] = nb[i] < 0 ? 0 : ((cd[i] >> 16) & 1 ? -1 : 1);
+    }
+  });
+}
+
+dgk_status dgk_rk4_timed(dgk_ctx* c, int nsteps, double dt, double* stage_ms, double* total_ms) {
+  if (!c || nsteps < 1) return fail(c, DGK_ERR_INVALID, "bad arguments");
+  return guarded(c, [&] {
+    std::vector<hipEvent_t> ev(8 * nsteps + 2);
+    for (auto& e : ev) DGK_HIP(hipEventCreate(&e));
+    try {
+      DGK_HIP(hipEventRecord(ev[8 * nsteps], c->stream));
+      for (int i = 0; i < nsteps; ++i) rk4_step(*c, dt, c->stream, &ev[8 * i]);
+      DGK_HIP(hipEventRecord(ev[8 * nsteps + 1], c->stream));
+      DGK_HIP(hipEventSynchronize(ev[8 * nsteps + 1]));
+      double sum = 0;
+      for (int i = 0; i < 4 * nsteps; ++i) {
+        float ms = 0;
+        DGK_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+        sum += ms;
+      }
+      float tot = 0;
+      DGK_HIP(hipEventElapsedTime(&tot, ev[8 * nsteps], ev[8 * nsteps + 1]));
+      if (stage_ms) *stage_ms = sum;
+      if (total_ms) *total_ms = tot;
+    } catch (...) {
+      for (auto& e : ev) (void)hipEventDestroy(e);
+      throw;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  });
+}
+
+const char* dgk_variant_name(int v) { return variant_name(v); }
+
+}  // extern "C"

@@ -1,0 +1,124 @@
+// Internal declarations shared by the libdgk translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/dgk.h"
+
+namespace dgk {
+
+#define DGK_HIP(call)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      throw DeviceError(std::string(#call) + ": " + hipGetErrorString(e_));                    \
+  } while (0)
+
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct MeshError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct Unsupported : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct InvalidArg : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+constexpr int kMaxNF = 4;
+constexpr int kMaxNFP = 21;
+constexpr int kMaxPerm = 6;
+
+// Operator tables for one (dim, order), in the precision of the kernels.
+struct DevOps {
+  void* W = nullptr;     // [dim][Np][Np]  W_r = Mhat^-1 S_r
+  void* LIFT = nullptr;  // [Np][Nf*Nfp]
+  int* fmask = nullptr;  // [Nf][Nfp]
+  int* vmap = nullptr;   // [Nf*nperm][Nfp] neighbour element node of own face node k
+};
+
+// Per-stage kernel arguments (precision-agnostic pointers).
+struct StageArgs {
+  int64_t K;
+  const void* Vin;
+  void* Vout;
+  void* U;
+  void* A;
+  const void* Jinv;       // [K][9]
+  const int32_t* nbr;     // [K*Nf]
+  const int32_t* code;    // [K*Nf]: (nbr_face*nperm + perm) | (penalty sign -1 ? 1<<16 : 0)
+  const void* W;
+  const void* LIFT;
+  const int* fmask;
+  const int* vmap;
+  double refn[kMaxNF][3];  // outward reference-face normals
+  double fsc[kMaxNF];      // reference face measure / face reference-element measure
+  double dt, c0, rho0, v0[3];
+  int stage;  // 0..3 RK4 stage, 4 = rhs (Vout = dt*L), 5 = forward Euler (Vout = Vin + dt*L)
+  int tiles_per_xcd;
+  int64_t tiles;
+};
+
+struct Ctx {
+  int dim = 0, p = 0, Np = 0, Nf = 0, Nfp = 0, nperm = 1;
+  int64_t K = 0, F = 0, Fb = 0;
+  int precision = DGK_FP64, sign_mode = DGK_SIGN_REFERENCE, device = 0, variant = 0;
+  double fc = 1;
+  double c0 = 1, rho0 = 1, v0[3] = {0, 0, 0};
+  hipStream_t stream = nullptr;
+  size_t tsize = 8;  // bytes per state value
+  // device buffers
+  void *U = nullptr, *A = nullptr, *V0 = nullptr, *V1 = nullptr, *Jinv = nullptr;
+  int32_t *nbr = nullptr, *code = nullptr, *nface = nullptr, *perm = nullptr;
+  double* stage_d = nullptr;  // FP64 staging for state transfer in FP32 mode
+  int* flag = nullptr;
+  DevOps ops;
+  double refn[kMaxNF][3] = {};
+  double fsc[kMaxNF] = {};
+  // sources
+  struct Src {
+    double amp, freq, phase, duration;
+    int* idx = nullptr;
+    int64_t n = 0;
+  };
+  std::vector<Src> sources;
+  int64_t device_bytes = 0;
+  // graph cache
+  hipGraphExec_t graph = nullptr;
+  int graph_steps = 0;
+  double graph_dt = 0;
+  int graph_euler = 0;
+  std::string err;
+  std::vector<void*> allocs;
+
+  void* alloc(size_t bytes) {
+    void* p = nullptr;
+    DGK_HIP(hipMalloc(&p, bytes));
+    allocs.push_back(p);
+    device_bytes += (int64_t)bytes;
+    return p;
+  }
+  ~Ctx();
+};
+
+// setup.hip
+void build_connectivity(Ctx& c, const double* vxyz_dev /* K*(dim+1)*3 */, const int32_t* vid_host /* K*(dim+1) */,
+                        const uint8_t* bc_first_host /* K*Nf: code of each face's first node */, double sigma);
+void build_geometry(Ctx& c, const double* vxyz_dev);
+// stage.hip
+void launch_stage(Ctx& c, StageArgs& a, hipStream_t s);
+void launch_source(Ctx& c, const Ctx::Src& s, double value, hipStream_t st);
+bool check_finite(Ctx& c);
+void convert_state(Ctx& c, const double* src_dev, void* dst_dev, int64_t n);       // FP64 -> state type
+void convert_state_back(Ctx& c, const void* src_dev, double* dst_dev, int64_t n);  // state type -> FP64
+const char* variant_name(int v);
+
+}  // namespace dgk
+
+struct dgk_ctx : dgk::Ctx {};

@@ -1,0 +1,295 @@
+"""Python binding of the dgalerkin MI355X libraries (ctypes, no torch types).
+
+libdgkhost.so : MSH ingestion, config semantics, mesh generators (include/dgk_host.h)
+libdgk.so     : the DG time-stepping hot path on gfx950 (include/dgk.h)
+
+The product path is the HIP library. If libdgk.so is missing or no GPU is
+usable, the solver entry points raise; there is no CPU fallback.
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+HOST_LIB = os.path.join(PKG_DIR, "libdgkhost.so")
+DEV_LIB = os.path.join(PKG_DIR, "libdgk.so")
+
+_host = None
+_dev = None
+P = ctypes.POINTER
+
+
+def _p(a, t=ctypes.c_double):
+    return a.ctypes.data_as(P(t))
+
+
+def host():
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_LIB):
+            raise RuntimeError(f"{HOST_LIB} is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(HOST_LIB)
+        L.dgh_last_error.restype = ctypes.c_char_p
+        for fn in ("dgh_read", "dgh_gen_rect", "dgh_gen_box", "dgh_gen_line"):
+            getattr(L, fn).restype = ctypes.c_void_p
+        L.dgh_read.argtypes = [ctypes.c_char_p]
+        L.dgh_gen_rect.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                   P(ctypes.c_int), ctypes.c_char_p]
+        L.dgh_gen_box.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_uint64, P(ctypes.c_int), P(ctypes.c_int), ctypes.c_int, P(ctypes.c_int),
+                                  ctypes.c_char_p]
+        L.dgh_gen_line.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, P(ctypes.c_int), ctypes.c_char_p]
+        L.dgh_free.argtypes = [ctypes.c_void_p]
+        L.dgh_write.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        L.dgh_top_info.argtypes = [ctypes.c_void_p, P(ctypes.c_int64)]
+        L.dgh_top_arrays.argtypes = [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_int64), P(ctypes.c_int64)]
+        L.dgh_config_json.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+        L.dgh_bc_groups.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int),
+                                    P(ctypes.c_int), P(ctypes.c_int64)]
+        L.dgh_group_nodes.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_int64)]
+        L.dgh_node_bc.argtypes = [ctypes.c_void_p, ctypes.c_char_p, P(ctypes.c_uint8)]
+        L.dgh_ref_element.argtypes = [ctypes.c_int, ctypes.c_int] + [P(ctypes.c_double)] * 6 + [P(ctypes.c_int)]
+        _host = L
+    return _host
+
+
+def _check_handle(h):
+    if not h:
+        raise RuntimeError(host().dgh_last_error().decode())
+    return ctypes.c_void_p(h)
+
+
+def ref_element(dim, p):
+    """Product reference-element operators (csrc/refelem.hpp) as numpy arrays."""
+    from math import comb
+    Np = comb(p + dim, dim)
+    Nf = dim + 1
+    Nfp = 1 if dim == 1 else comb(p + dim - 1, dim - 1)
+    out = dict(nodes=np.zeros((Np, 3)), M=np.zeros((Np, Np)), S=np.zeros((dim, Np, Np)), Mf=np.zeros((Nfp, Nfp)),
+               W=np.zeros((dim, Np, Np)), LIFT=np.zeros((Np, Nf * Nfp)), fmask=np.zeros((Nf, Nfp), np.int32))
+    rc = host().dgh_ref_element(dim, p, *[_p(out[k]) for k in ("nodes", "M", "S", "Mf", "W", "LIFT")],
+                                _p(out["fmask"], ctypes.c_int))
+    if rc:
+        raise ValueError(host().dgh_last_error().decode())
+    return out
+
+
+def parse_config(text):
+    buf = ctypes.create_string_buffer(1 << 20)
+    rc = host().dgh_config_json(text.encode(), buf, len(buf))
+    if rc:
+        raise ValueError(host().dgh_last_error().decode())
+    return json.loads(buf.value.decode())
+
+
+class Mesh:
+    """The element block the solver works on plus its boundary groups (host arrays)."""
+
+    def __init__(self, handle, config_text):
+        L = host()
+        self.h = handle
+        info = np.zeros(4, np.int64)
+        L.dgh_top_info(self.h, _p(info, ctypes.c_int64))
+        self.dim, self.order, self.K, self.Np = (int(x) for x in info)
+        self.node_xyz = np.zeros((self.K * self.Np, 3))
+        self.node_tags = np.zeros(self.K * self.Np, np.int64)
+        self.elem_tags = np.zeros(self.K, np.int64)
+        L.dgh_top_arrays(self.h, _p(self.node_xyz), _p(self.node_tags, ctypes.c_int64),
+                         _p(self.elem_tags, ctypes.c_int64))
+        self.config_text = config_text
+        self.node_bc = np.zeros(self.K * self.Np, np.uint8)
+        if L.dgh_node_bc(self.h, config_text.encode(), _p(self.node_bc, ctypes.c_uint8)):
+            raise ValueError(L.dgh_last_error().decode())
+        ng = ctypes.c_int(0)
+        tags = np.zeros(64, np.int32)
+        refl = np.zeros(64, np.int32)
+        sizes = np.zeros(64, np.int64)
+        L.dgh_bc_groups(self.h, config_text.encode(), 64, ctypes.byref(ng), _p(tags, ctypes.c_int32),
+                        _p(refl, ctypes.c_int32), _p(sizes, ctypes.c_int64))
+        self.bc_groups = []
+        for i in range(ng.value):
+            nodes = np.zeros(int(sizes[i]), np.int64)
+            L.dgh_group_nodes(self.h, int(tags[i]), _p(nodes, ctypes.c_int64))
+            self.bc_groups.append((int(tags[i]), int(refl[i]), nodes))
+
+    def write(self, path):
+        if host().dgh_write(self.h, path.encode()):
+            raise RuntimeError(host().dgh_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None:
+            host().dgh_free(self.h)
+            self.h = None
+
+
+def read_mesh(path, config_text):
+    return Mesh(_check_handle(host().dgh_read(path.encode())), config_text)
+
+
+def gen_rect(nx, ny, Lx, Ly, order, side_tags, names, config_text):
+    st = np.array(side_tags, np.int32)
+    return Mesh(_check_handle(host().dgh_gen_rect(nx, ny, Lx, Ly, order, _p(st, ctypes.c_int), names.encode())),
+                config_text)
+
+
+def gen_box(nx, ny, nz, Lx, Ly, Lz, order, jitter, seed, side_tags, names, config_text, block=None, block_tag=0):
+    st = np.array(side_tags, np.int32)
+    b0 = np.array(block[0] if block else (0, 0, 0), np.int32)
+    b1 = np.array(block[1] if block else (0, 0, 0), np.int32)
+    h = host().dgh_gen_box(nx, ny, nz, Lx, Ly, Lz, order, jitter, seed, _p(b0, ctypes.c_int), _p(b1, ctypes.c_int),
+                           block_tag, _p(st, ctypes.c_int), names.encode())
+    return Mesh(_check_handle(h), config_text)
+
+
+def gen_line(n, L, order, side_tags, names, config_text):
+    st = np.array(side_tags, np.int32)
+    return Mesh(_check_handle(host().dgh_gen_line(n, L, order, _p(st, ctypes.c_int), names.encode())), config_text)
+
+
+# ----------------------------------------------------------------------------- device library
+FP64, FP32 = 0, 1
+SIGN_REFERENCE, SIGN_STABLE = 0, 1
+
+
+class _Mesh(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("order", ctypes.c_int), ("num_elements", ctypes.c_int64),
+                ("node_xyz", P(ctypes.c_double)), ("node_tags", P(ctypes.c_int64)), ("node_bc", P(ctypes.c_uint8))]
+
+
+class _Phys(ctypes.Structure):
+    _fields_ = [("c0", ctypes.c_double), ("rho0", ctypes.c_double), ("v0", ctypes.c_double * 3)]
+
+
+class _Opts(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("flux_sign", ctypes.c_int), ("device", ctypes.c_int),
+                ("variant", ctypes.c_int), ("reserved", ctypes.c_int * 6)]
+
+
+class _Info(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("order", ctypes.c_int), ("Np", ctypes.c_int), ("Nf", ctypes.c_int),
+                ("Nfp", ctypes.c_int), ("num_elements", ctypes.c_int64), ("num_faces", ctypes.c_int64),
+                ("num_boundary_faces", ctypes.c_int64), ("precision", ctypes.c_int), ("variant", ctypes.c_int),
+                ("fc", ctypes.c_double), ("device_bytes", ctypes.c_int64)]
+
+
+DEV_SYMBOLS = ("dgk_create", "dgk_destroy", "dgk_last_error", "dgk_info_get", "dgk_set_state", "dgk_get_state",
+               "dgk_set_sources", "dgk_rk4", "dgk_euler", "dgk_rhs", "dgk_sync", "dgk_check_finite",
+               "dgk_connectivity", "dgk_rk4_timed", "dgk_variant_name")
+
+
+def device():
+    """Load libdgk.so. Raises when it is not built: the solver has no CPU fallback."""
+    global _dev
+    if _dev is None:
+        if not os.path.exists(DEV_LIB):
+            raise RuntimeError(f"{DEV_LIB} is not built (run __graft_entry__.build()); there is no CPU fallback")
+        L = ctypes.CDLL(DEV_LIB)
+        vp = ctypes.c_void_p
+        L.dgk_create.argtypes = [P(_Mesh), P(_Phys), P(_Opts), P(vp)]
+        L.dgk_destroy.argtypes = [vp]
+        L.dgk_last_error.argtypes = [vp]
+        L.dgk_last_error.restype = ctypes.c_char_p
+        L.dgk_info_get.argtypes = [vp, P(_Info)]
+        L.dgk_set_state.argtypes = [vp, P(ctypes.c_double)]
+        L.dgk_get_state.argtypes = [vp, P(ctypes.c_double)]
+        L.dgk_set_sources.argtypes = [vp, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double)]
+        L.dgk_rk4.argtypes = [vp, ctypes.c_int, ctypes.c_double, P(ctypes.c_double)]
+        L.dgk_euler.argtypes = [vp, ctypes.c_int, ctypes.c_double, P(ctypes.c_double)]
+        L.dgk_rhs.argtypes = [vp, P(ctypes.c_double), P(ctypes.c_double), ctypes.c_double]
+        L.dgk_sync.argtypes = [vp]
+        L.dgk_check_finite.argtypes = [vp]
+        L.dgk_connectivity.argtypes = [vp] + [P(ctypes.c_int32)] * 4
+        L.dgk_rk4_timed.argtypes = [vp, ctypes.c_int, ctypes.c_double, P(ctypes.c_double), P(ctypes.c_double)]
+        L.dgk_variant_name.argtypes = [ctypes.c_int]
+        L.dgk_variant_name.restype = ctypes.c_char_p
+        _dev = L
+    return _dev
+
+
+class DGKError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"dgk status {status}: {msg}")
+        self.status = status
+
+
+class Solver:
+    """One libdgk context: the reference's Mesh + solver state on the GPU."""
+
+    def __init__(self, mesh, c0, rho0, v0=(0.0, 0.0, 0.0), precision=FP64, flux_sign=SIGN_REFERENCE, device_id=0,
+                 variant=0):
+        L = device()
+        self.mesh = mesh
+        self.K, self.Np = mesh.K, mesh.Np
+        self._xyz = np.ascontiguousarray(mesh.node_xyz, dtype=np.float64)
+        self._tags = np.ascontiguousarray(mesh.node_tags, dtype=np.int64)
+        self._bc = np.ascontiguousarray(mesh.node_bc, dtype=np.uint8)
+        m = _Mesh(mesh.dim, mesh.order, mesh.K, _p(self._xyz), _p(self._tags, ctypes.c_int64),
+                  _p(self._bc, ctypes.c_uint8))
+        ph = _Phys(float(c0), float(rho0), (ctypes.c_double * 3)(*[float(x) for x in v0]))
+        op = _Opts(int(precision), int(flux_sign), int(device_id), int(variant))
+        h = ctypes.c_void_p()
+        st = L.dgk_create(ctypes.byref(m), ctypes.byref(ph), ctypes.byref(op), ctypes.byref(h))
+        if st:
+            raise DGKError(st, L.dgk_last_error(None).decode())
+        self.h = h
+        self.info = _Info()
+        L.dgk_info_get(self.h, ctypes.byref(self.info))
+
+    def _check(self, st):
+        if st:
+            raise DGKError(st, device().dgk_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            device().dgk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_state(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert u.size == 4 * self.K * self.Np
+        self._check(device().dgk_set_state(self.h, _p(u)))
+
+    def get_state(self):
+        u = np.zeros((4, self.K * self.Np))
+        self._check(device().dgk_get_state(self.h, _p(u)))
+        return u
+
+    def set_sources(self, sources):
+        src = np.ascontiguousarray(sources, dtype=np.float64).reshape(-1, 9)
+        self._check(device().dgk_set_sources(self.h, len(src), _p(src), _p(self._xyz)))
+
+    def rk4(self, nsteps, dt, t_start=None):
+        ts = None if t_start is None else np.ascontiguousarray(t_start, dtype=np.float64)
+        self._check(device().dgk_rk4(self.h, int(nsteps), float(dt), None if ts is None else _p(ts)))
+
+    def euler(self, nsteps, dt, t_start=None):
+        ts = None if t_start is None else np.ascontiguousarray(t_start, dtype=np.float64)
+        self._check(device().dgk_euler(self.h, int(nsteps), float(dt), None if ts is None else _p(ts)))
+
+    def rhs(self, u, dt):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros_like(u)
+        self._check(device().dgk_rhs(self.h, _p(u), _p(out), float(dt)))
+        return out
+
+    def sync(self):
+        self._check(device().dgk_sync(self.h))
+
+    def check_finite(self):
+        return device().dgk_check_finite(self.h) == 0
+
+    def connectivity(self):
+        n = self.K * self.info.Nf
+        arrs = [np.zeros(n, np.int32) for _ in range(4)]
+        self._check(device().dgk_connectivity(self.h, *[_p(a, ctypes.c_int32) for a in arrs]))
+        return dict(zip(("nbr", "nface", "perm", "sign"), (a.reshape(self.K, -1) for a in arrs)))
+
+    def rk4_timed(self, nsteps, dt):
+        s, t = ctypes.c_double(), ctypes.c_double()
+        self._check(device().dgk_rk4_timed(self.h, int(nsteps), float(dt), ctypes.byref(s), ctypes.byref(t)))
+        return s.value, t.value

@@ -1,0 +1,122 @@
+/* libdgk: the DG time-stepping hot path of povanberg/DGFEM-Acoustic on gfx950.
+ *
+ * Drop-in boundary for the reference's in-process solver API:
+ *   dgk_create       Mesh::Mesh connectivity/geometry/BC setup (src/Mesh.cpp:20-435)
+ *                    + Mesh::precomputeMassMatrix (src/Mesh.cpp:440-466)
+ *   dgk_rk4          the time-step body of solver::rungeKutta (src/solver.cpp:247-285):
+ *                    hard source overwrite, then classic RK4 with numStep(beta=0) stages
+ *   dgk_euler        the time-step body of solver::forwardEuler (src/solver.cpp:139-153)
+ *   dgk_rhs          one Mesh::updateFlux + solver::numStep(beta=0)
+ *                    (src/Mesh.cpp:569-674, src/solver.cpp:35-52): out = dt*Minv*(S-F)
+ *   dgk_set_state /
+ *   dgk_get_state    the caller-owned state u[4][K*Np] (src/dgalerkin.cpp:36), same layout:
+ *                    equation-major (p, vx, vy, vz), then element, then Gmsh element node
+ * Conventions: plain C types only; every call returns a dgk_status; the library owns
+ * device memory, the caller owns host buffers; calls on one context are ordered on the
+ * context's HIP stream and dgk_get_state / dgk_sync block until prior work is done.
+ * One context must not be used by two host threads at once.
+ */
+#ifndef DGK_H
+#define DGK_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  DGK_OK = 0,
+  DGK_ERR_INVALID = 1,      /* bad argument */
+  DGK_ERR_UNSUPPORTED = 2,  /* element type/order or geometry the kernels do not handle */
+  DGK_ERR_MESH = 3,         /* inconsistent mesh (non-conforming, non-manifold, inverted) */
+  DGK_ERR_DEVICE = 4,       /* HIP runtime failure or missing GPU */
+  DGK_ERR_NONFINITE = 5     /* state check found NaN/Inf */
+} dgk_status;
+
+enum { DGK_FP64 = 0, DGK_FP32 = 1 };
+/* Interior penalty sign. REFERENCE reproduces the reference's fc * orientation rule
+ * (src/Mesh.cpp:210-211, 278-294, 519-527) with the normal sign taken at the first point of
+ * the face quadrature rule (Gauss-Legendre, order+2 points, collapsed on triangles).
+ * STABLE uses the dissipative (upwind-consistent) sign on every interior face. */
+enum { DGK_SIGN_REFERENCE = 0, DGK_SIGN_STABLE = 1 };
+
+typedef struct {
+  int dim;                   /* element dimension 1..3 */
+  int order;                 /* Lagrange order 1..5 */
+  int64_t num_elements;      /* K */
+  const double* node_xyz;    /* [K*Np*3] element node coordinates, Gmsh node order */
+  const int64_t* node_tags;  /* [K*Np] node tags; conforming neighbours share tags */
+  const uint8_t* node_bc;    /* [K*Np] 1 reflecting / 0 absorbing; a boundary face takes
+                                the code of its first node (src/Mesh.cpp:364-384). NULL = all absorbing */
+} dgk_mesh;
+
+typedef struct {
+  double c0, rho0;
+  double v0[3];
+} dgk_phys;
+
+typedef struct {
+  int precision;   /* DGK_FP64 (drop-in) or DGK_FP32 (FP32 arithmetic, FP64 geometry rounded once) */
+  int flux_sign;   /* DGK_SIGN_REFERENCE or DGK_SIGN_STABLE */
+  int device;      /* HIP device ordinal */
+  int variant;     /* stage-kernel variant, 0 = autotuned/default (see dgk_variant_name) */
+  int reserved[6];
+} dgk_opts;
+
+typedef struct {
+  int dim, order, Np, Nf, Nfp;
+  int64_t num_elements, num_faces, num_boundary_faces;
+  int precision, variant;
+  double fc;              /* the reference's flux coefficient for this element type */
+  int64_t device_bytes;   /* device memory held by the context */
+} dgk_info;
+
+typedef struct dgk_ctx dgk_ctx;
+
+/* Context lifecycle. On failure *out is NULL and dgk_last_error(NULL) describes it. */
+dgk_status dgk_create(const dgk_mesh* mesh, const dgk_phys* phys, const dgk_opts* opts, dgk_ctx** out);
+dgk_status dgk_destroy(dgk_ctx* ctx);
+const char* dgk_last_error(const dgk_ctx* ctx);
+dgk_status dgk_info_get(const dgk_ctx* ctx, dgk_info* info);
+
+/* State transfer, u[4][K*Np] FP64 host arrays. */
+dgk_status dgk_set_state(dgk_ctx* ctx, const double* u);
+dgk_status dgk_get_state(dgk_ctx* ctx, double* u);
+
+/* Point sources (src/configParser.cpp:65-102): n rows of 9 doubles
+ * {pole, x, y, z, radius, amp, freq, phase, duration}. At the start of every step with
+ * t < duration, p is overwritten with amp*sin(2*pi*freq*t + phase) at the element nodes
+ * strictly inside the radius (src/solver.cpp:197-210, 248-256); later rows win where
+ * sources overlap. node_xyz [K*Np*3] are the element node coordinates (as in dgk_mesh). */
+dgk_status dgk_set_sources(dgk_ctx* ctx, int n, const double* src, const double* node_xyz);
+
+/* Advance nsteps RK4 (or forward-Euler) steps of size dt. t_start[i] is the time at the
+ * start of step i (used by the sources; may be NULL when none are set). Asynchronous. */
+dgk_status dgk_rk4(dgk_ctx* ctx, int nsteps, double dt, const double* t_start);
+dgk_status dgk_euler(dgk_ctx* ctx, int nsteps, double dt, const double* t_start);
+
+/* out = dt * L(u): one updateFlux + numStep(beta = 0), host arrays [4][K*Np]. */
+dgk_status dgk_rhs(dgk_ctx* ctx, const double* u, double* out, double dt);
+
+dgk_status dgk_sync(dgk_ctx* ctx);
+/* Returns DGK_ERR_NONFINITE if the device state holds NaN/Inf. */
+dgk_status dgk_check_finite(dgk_ctx* ctx);
+
+/* Face connectivity built on the device, per element face e*Nf+j:
+ *   nbr[e*Nf+j]   neighbour element, or -1 (absorbing boundary) / -2 (reflecting boundary)
+ *   nface[e*Nf+j] neighbour's local face (-1 on the boundary)
+ *   perm[e*Nf+j]  index of the vertex permutation taking this face's vertex order to the
+ *                 neighbour's (-1 on the boundary)
+ *   sign[e*Nf+j]  interior penalty sign (+1/-1; 0 on the boundary) */
+dgk_status dgk_connectivity(dgk_ctx* ctx, int32_t* nbr, int32_t* nface, int32_t* perm, int32_t* sign);
+
+/* Timed run for measurement: nsteps RK4 steps with a HIP event pair around every stage
+ * launch on the context's stream. Writes the summed stage-kernel time and the wall time of
+ * the whole sequence (both ms, device clock). */
+dgk_status dgk_rk4_timed(dgk_ctx* ctx, int nsteps, double dt, double* stage_ms, double* total_ms);
+
+const char* dgk_variant_name(int variant);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,74 @@
+"""The C-ABI libraries load and export every symbol their headers declare; argument
+validation that happens before any device call is exercised without a GPU."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import dgk
+from workloads import conf
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dg[kh]_[a-z0-9_]+)\s*\(", text)))
+
+
+def ensure_device_lib():
+    if not os.path.exists(dgk.DEV_LIB):
+        subprocess.run([sys.executable, os.path.join(dgk.PKG_DIR, "build.py"), "device"], check=True)
+
+
+def test_libdgk_exports_header():
+    ensure_device_lib()
+    lib = ctypes.CDLL(dgk.DEV_LIB)
+    names = declared("dgk.h")
+    assert "dgk_create" in names and "dgk_rk4" in names
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(dgk.DEV_SYMBOLS)
+
+
+def test_libdgkhost_exports_header():
+    lib = ctypes.CDLL(dgk.HOST_LIB)
+    for n in declared("dgk_host.h"):
+        if n == "dgh_node_bc_resolve":
+            continue
+        assert hasattr(lib, n), n
+
+
+def _create(dim, order, K, xyz, tags, bc=None):
+    ensure_device_lib()
+    L = dgk.device()
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    tags = np.ascontiguousarray(tags, np.int64)
+    m = dgk._Mesh(dim, order, K, dgk._p(xyz), dgk._p(tags, ctypes.c_int64), None)
+    ph = dgk._Phys(340.0, 1.225, (ctypes.c_double * 3)(0, 0, 0))
+    op = dgk._Opts(0, 0, 0, 0)
+    h = ctypes.c_void_p()
+    st = L.dgk_create(ctypes.byref(m), ctypes.byref(ph), ctypes.byref(op), ctypes.byref(h))
+    return st, L.dgk_last_error(None).decode(), h
+
+
+def test_invalid_arguments_fail_before_device_use():
+    st, msg, h = _create(2, 6, 1, np.zeros(3 * 28), np.arange(28))
+    assert st == 2 and "order" in msg and not h
+    st, msg, h = _create(4, 1, 1, np.zeros(12), np.arange(4))
+    assert st == 2 and not h
+    st, msg, h = _create(2, 1, 0, np.zeros(9), np.arange(3))
+    assert st == 1 and not h
+
+
+def test_curved_element_is_rejected():
+    m = dgk.gen_rect(1, 1, 1.0, 1.0, 2, [1, 1, 1, 1], "1:Wall", conf("Wall=Reflecting\n"))
+    xyz = m.node_xyz.copy()
+    xyz[3, 1] += 0.05  # bend the first edge node of element 0
+    st, msg, h = _create(2, 2, m.K, xyz, m.node_tags)
+    assert st == 2 and "curved" in msg and not h

@@ -1,0 +1,177 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+FP64 bar: relative L2 <= 1e-13 for one operator application and <= 1e-12 after N steps
+(the drop-in tolerance). FP32 bar: relative L2 <= 1e-5 against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import dgk
+import oracle_binding as ob
+import workloads as wl
+from workloads import conf
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def mixed_mesh(dim, p, n=None):
+    bcs = "Wall=Reflecting\nAbs=Absorbing\n"
+    if dim == 1:
+        return dgk.gen_line(n or 9, 1.0, p, [1, 2], "1:Wall;2:Abs", conf(bcs))
+    if dim == 2:
+        return dgk.gen_rect(n or 6, n or 5, 1.0, 0.8, p, [1, 2, 1, 2], "1:Wall;2:Abs", conf(bcs))
+    return dgk.gen_box(n or 3, 2, 2, 1.0, 0.8, 0.7, p, 0.1, 2, [1, 1, 2, 2, 1, 2], "1:Wall;2:Abs", conf(bcs))
+
+
+CASES = [(d, p) for d in (1, 2, 3) for p in (1, 2, 3, 4, 5)]
+
+
+@pytest.mark.parametrize("dim,p", CASES)
+@pytest.mark.parametrize("sign", [dgk.SIGN_REFERENCE, dgk.SIGN_STABLE])
+def test_rhs_and_rk4_fp64(dim, p, sign):
+    print(f"case dim={dim} p={p} sign={sign}", flush=True)
+    m = mixed_mesh(dim, p)
+    v0 = (40.0, -20.0, 10.0)
+    o = ob.Oracle(m, 340.0, 1.225, v0=v0, sign_mode=sign)
+    s = dgk.Solver(m, 340.0, 1.225, v0=v0, flux_sign=sign)
+    u = np.random.default_rng(11).standard_normal((4, m.K * m.Np))
+    assert rel(s.rhs(u, 1e-4), o.rhs(u, 1e-4)) <= 1e-13
+    s.set_state(u)
+    s.rk4(40, 2e-6)  # one 25-step graph launch plus 15 direct steps
+    assert rel(s.get_state(), o.step(u, 40, 2e-6)) <= 1e-12
+
+
+def test_connectivity_equals_oracle_table():
+    for m in (mixed_mesh(2, 3), mixed_mesh(3, 3), wl.cube_3d(4, 2)[0]):
+        o = ob.Oracle(m, 340.0, 1.225)
+        s = dgk.Solver(m, 340.0, 1.225)
+        c = o.connectivity()
+        g = s.connectivity()
+        r = dgk.ref_element(m.dim, m.order)
+        Nf, Nfp = r["fmask"].shape
+        for e in range(m.K):
+            for j in range(Nf):
+                f = c["elFIds"][e, j]
+                up, dn = c["fNbr"][f]
+                if dn < 0:
+                    assert g["nbr"][e, j] == (-2 if c["fBC"][f] == 1 else -1)
+                    continue
+                other = dn if e == up else up
+                assert g["nbr"][e, j] == other
+                jn = g["nface"][e, j]
+                assert c["elFIds"][other, jn] == f
+                # node pairing: oracle maps by tag, device by lattice permutation
+                side = 0 if e == up else 1
+                pairs_o = {(int(c["fNToElNIds"][f, k, side]), int(c["fNToElNIds"][f, k, 1 - side])) for k in range(Nfp)}
+                own = r["fmask"][j]
+                tags_e = m.node_tags[e * m.Np:(e + 1) * m.Np]
+                tags_n = m.node_tags[other * m.Np:(other + 1) * m.Np]
+                pairs_g = set()
+                for k in range(Nfp):
+                    a = int(own[k])
+                    b = int(np.where(tags_n == tags_e[a])[0][0])
+                    pairs_g.add((a, b))
+                assert pairs_o == pairs_g
+                # interior penalty sign: fc * orientation of the lower-index element
+                jup = list(c["elFIds"][up]).index(f)
+                assert g["sign"][e, j] == o.fc * c["elFOrientation"][up, jup]
+
+
+def test_square_2d_1000_steps_fp64():
+    """The 2D P3 workload (3200 elements, reflecting walls, Gaussian pulse) for 1000 RK4 steps."""
+    m, ph, dt, u0 = wl.square_2d()
+    o = ob.Oracle(m, ph["c0"], ph["rho0"], factorized_only=True)
+    s = dgk.Solver(m, ph["c0"], ph["rho0"])
+    s.set_state(u0)
+    s.rk4(1000, dt)
+    g = s.get_state()
+    c = o.step(u0, 1000, dt)
+    for q in range(3):
+        assert rel(g[q], c[q]) <= 1e-12, q
+    assert np.abs(g[3]).max() == 0.0
+
+
+def test_cube_3d_small_both_signs_fp64():
+    for sign in (dgk.SIGN_REFERENCE, dgk.SIGN_STABLE):
+        m, ph, dt, u0 = wl.cube_3d(6, 3)
+        o = ob.Oracle(m, ph["c0"], ph["rho0"], v0=ph["v0"], sign_mode=sign)
+        s = dgk.Solver(m, ph["c0"], ph["rho0"], v0=ph["v0"], flux_sign=sign)
+        s.set_state(u0)
+        s.rk4(200, dt)
+        assert rel(s.get_state(), o.step(u0, 200, dt)) <= 1e-12
+
+
+def test_euler_and_sources_fp64():
+    m = mixed_mesh(2, 2, 8)
+    o = ob.Oracle(m, 340.0, 1.225, v0=(10.0, 0.0, 0.0))
+    s = dgk.Solver(m, 340.0, 1.225, v0=(10.0, 0.0, 0.0))
+    src = np.array([[0, 0.5, 0.4, 0, 0.2, 1.0, 800.0, 0.3, 1e-4],
+                    [1, 0.3, 0.4, 0, 0.1, 2.0, 500.0, 0.0, 1.0],
+                    [1, 0.5, 0.4, 0, 0.1, 2.0, 500.0, np.pi, 1.0]])
+    dt = 1e-6
+    ts = np.cumsum(np.full(60, dt)) - dt
+    u0 = np.zeros((4, m.K * m.Np))
+    s.set_sources(src)
+    s.set_state(u0)
+    s.rk4(60, dt, ts)
+    assert rel(s.get_state(), o.step(u0, 60, dt, t_steps=ts, sources=src)) <= 1e-12
+    s.set_state(u0)
+    s.euler(30, dt, ts[:30])
+    assert rel(s.get_state(), o.step(u0, 30, dt, t_steps=ts[:30], sources=src, euler=True)) <= 1e-12
+
+
+def test_cube_3d_full_size_fp64():
+    """The 3D P3 workload at full size (196,608 tets): 3 steps against the oracle, then finite."""
+    m, ph, dt, u0 = wl.cube_3d()
+    assert m.K == 196608
+    s = dgk.Solver(m, ph["c0"], ph["rho0"], v0=ph["v0"], flux_sign=dgk.SIGN_STABLE)
+    o = ob.Oracle(m, ph["c0"], ph["rho0"], v0=ph["v0"], sign_mode=ob.SIGN_STABLE, factorized_only=True)
+    s.set_state(u0)
+    s.rk4(3, dt)
+    assert rel(s.get_state(), o.step(u0, 3, dt)) <= 1e-13
+    assert s.check_finite()
+
+
+def test_nonfinite_state_is_reported():
+    m = mixed_mesh(2, 1)
+    s = dgk.Solver(m, 340.0, 1.225)
+    u = np.zeros((4, m.K * m.Np))
+    s.set_state(u)
+    assert s.check_finite()
+    u[1, 5] = np.nan
+    s.set_state(u)
+    assert not s.check_finite()
+
+
+def test_nonconforming_mesh_is_rejected():
+    m = mixed_mesh(2, 2)
+    tags = m.node_tags.copy()
+    # give element 0's edge node a tag nobody shares: its face no longer matches its neighbour
+    tags[3] = tags.max() + 1
+    m.node_tags = tags
+    with pytest.raises(dgk.DGKError) as ei:
+        dgk.Solver(m, 340.0, 1.225)
+    assert ei.value.status == 3
+
+
+@pytest.mark.parametrize("workload", ["square", "cube"])
+def test_fp32_error_bound(workload):
+    if workload == "square":
+        m, ph, dt, u0 = wl.square_2d()
+        n = 1000
+    else:
+        m, ph, dt, u0 = wl.cube_3d(8, 3)
+        n = 300
+    o = ob.Oracle(m, ph["c0"], ph["rho0"], v0=ph["v0"], sign_mode=ob.SIGN_STABLE)
+    s = dgk.Solver(m, ph["c0"], ph["rho0"], v0=ph["v0"], precision=dgk.FP32, flux_sign=dgk.SIGN_STABLE)
+    s.set_state(u0)
+    s.rk4(n, dt)
+    g = s.get_state()
+    c = o.step(u0, n, dt)
+    errs = [rel(g[q], c[q]) for q in range(m.dim + 1)]
+    print("fp32 rel-L2 per field", workload, errs)
+    assert max(errs) <= 1e-5

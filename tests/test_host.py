@@ -1,0 +1,176 @@
+"""Host side of the boundary: config semantics, MSH ingestion, generators, BC rule."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import dgk
+import oracle_binding as ob
+from workloads import conf
+
+
+# ----------------------------------------------------------------------------- config
+def test_config_semantics():
+    text = ("[Config]\n# comment line\n  timeStart = 0\ntimeEnd=0.1\ntimeStep=1e-4\ntimeRate=0.001\n"
+            "elementType = Lagrange\ntimeIntMethod=Runge-Kutta\nnumThreads=1\nv0_x = 1\nv0_y=2\nv0_z=3\n"
+            "rho0=1.225\nc0 = 343.3\nsaveFile = out file.msh\n"
+            "source1 = monopole, 0,0,0, 0.5, 1,100,0,0.05\n"
+            "source2 = dipole, 1,2,3, 0.2, 2,50,0.5,1\n"
+            "source3 = quadrupole, 0,0,0, 0.4, 3,10,0,1\n"
+            "initialCondtition1 = gaussian, 0.5,0.5,0,0.005,1\n"
+            "timeEnd=0.2\n")
+    c = dgk.parse_config(text)
+    assert c["timeEnd"] == 0.2  # later duplicate wins (std::map assignment)
+    assert c["numThreads"] == 0  # 1 maps to 0 (src/configParser.cpp:58)
+    assert c["saveFile"] == "outfile.msh"  # all whitespace removed
+    assert c["v0"] == [1, 2, 3] and c["c0"] == 343.3
+    s = c["sources"]
+    assert s[0] == [0, 0, 0, 0, 0.5, 1, 100, 0, 0.05]
+    assert s[1] == [1, 1 - 0.2, 2, 3, 0.1, 2, 50, 0.5, 1]
+    assert s[2] == pytest.approx([1, 1 + 0.2, 2, 3, 0.1, 2, 50, 0.5 + np.pi, 1])
+    assert [q[1:4] for q in s[3:7]] == [[-0.4, 0, 0], [0.4, 0, 0], [0, -0.4, 0], [0, 0.4, 0]]
+    assert [q[7] for q in s[3:7]] == pytest.approx([0, 0, np.pi, np.pi])
+    assert c["initConditions"] == [[0, 0.5, 0.5, 0, 0.005, 1]]
+
+
+def test_config_missing_required_key_is_an_error():
+    text = "timeStart=0\ntimeEnd=1\ntimeStep=0.1\ntimeRate=1\nnumThreads=2\nv0_x=0\nv0_y=0\nv0_z=0\nrho0=1\n"
+    with pytest.raises(ValueError, match="c0"):
+        dgk.parse_config(text)
+    with pytest.raises(ValueError, match="gpu.precision"):
+        dgk.parse_config(text + "c0=1\ngpu.precision=fp16\n")
+
+
+def test_reference_config_files(reference_dir):
+    c = dgk.parse_config(open(os.path.join(reference_dir, "doc/config/config.conf")).read())
+    assert (c["timeStart"], c["timeEnd"], c["timeStep"], c["timeRate"]) == (0, 0.1, 0.0001, 0.001)
+    assert c["numThreads"] == 4 and c["c0"] == 343.3 and c["rho0"] == 1.225
+    assert c["sources"] == [] and c["initConditions"] == [[0, 0, 0, 0, 1, 1]]
+    c = dgk.parse_config(open(os.path.join(reference_dir, "doc/config/Room_2D.conf")).read())
+    assert c["sources"] == [[0, 3, 1.2, 0, 0.1, 10, 1500, 0, 0.01]]
+    c = dgk.parse_config(open(os.path.join(reference_dir, "doc/config/Amphi_Pulse_3D.conf")).read())
+    assert c["initConditions"] == [[0, 6.5, 6.5, 1.8, 1, 0.05]] and c["timeStep"] == 1e-5
+
+
+# ----------------------------------------------------------------------------- MSH
+@pytest.mark.parametrize("rel,dim,K", [("doc/1d/line.msh", 1, 100), ("doc/2d/square.msh", 2, None),
+                                       ("doc/3d/cube.msh", 3, None), ("doc/2d/room.msh", 2, None)])
+def test_read_reference_meshes_and_roundtrip(reference_dir, tmp_path, rel, dim, K):
+    cfg = conf("Absorbing=Absorbing\nReflecting=Reflecting\nBoundary=Absorbing\n")
+    m = dgk.read_mesh(os.path.join(reference_dir, rel), cfg)
+    assert m.dim == dim and m.order == 1
+    if K:
+        assert m.K == K
+    X = m.node_xyz.reshape(m.K, m.Np, 3)
+    assert np.isfinite(X).all()
+    out = str(tmp_path / "rt.msh")
+    m.write(out)
+    m2 = dgk.read_mesh(out, cfg)
+    assert (m2.node_tags == m.node_tags).all() and np.array_equal(m2.node_xyz, m.node_xyz)
+    assert [(g[0], g[1], len(g[2])) for g in m2.bc_groups] == [(g[0], g[1], len(g[2])) for g in m.bc_groups]
+    # the oracle builds a consistent face table on the reference's own mesh
+    o = ob.Oracle(m, 343.0, 1.225)
+    assert 2 * o.F - o.Fb == m.K * o.Nf
+
+
+def _two_triangles_v22(path, binary=False):
+    nodes = [(1, 0.0, 0.0), (2, 1.0, 0.0), (3, 1.0, 1.0), (4, 0.0, 1.0)]
+    with open(path, "w") as f:
+        f.write('$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$PhysicalNames\n2\n1 7 "Wall"\n2 9 "Domain"\n'
+                "$EndPhysicalNames\n$Nodes\n4\n")
+        for t, x, y in nodes:
+            f.write(f"{t} {x} {y} 0\n")
+        f.write("$EndNodes\n$Elements\n6\n")
+        f.write("1 1 2 7 1 1 2\n2 1 2 7 1 2 3\n3 1 2 7 1 3 4\n4 1 2 7 1 4 1\n")
+        f.write("5 2 2 9 5 1 2 3\n6 2 2 9 5 1 3 4\n$EndElements\n")
+
+
+def _two_triangles_v41_binary(path):
+    b = bytearray()
+    b += b"$MeshFormat\n4.1 1 8\n" + struct.pack("<i", 1) + b"\n$EndMeshFormat\n"
+    b += b'$PhysicalNames\n2\n1 7 "Wall"\n2 9 "Domain"\n$EndPhysicalNames\n'
+    b += b"$Entities\n" + struct.pack("<4Q", 0, 1, 1, 0)
+    b += struct.pack("<i6dQiQ", 1, 0, 0, 0, 1, 1, 0, 1, 7, 0)
+    b += struct.pack("<i6dQiQ", 5, 0, 0, 0, 1, 1, 0, 1, 9, 0)
+    b += b"\n$EndEntities\n$Nodes\n" + struct.pack("<4Q", 1, 4, 1, 4) + struct.pack("<iiiQ", 2, 5, 0, 4)
+    b += struct.pack("<4Q", 1, 2, 3, 4) + struct.pack("<12d", 0, 0, 0, 1, 0, 0, 1, 1, 0, 0, 1, 0)
+    b += b"\n$EndNodes\n$Elements\n" + struct.pack("<4Q", 2, 6, 1, 6)
+    b += struct.pack("<iiiQ", 1, 1, 1, 4) + struct.pack("<12Q", 1, 1, 2, 2, 2, 3, 3, 3, 4, 4, 4, 1)
+    b += struct.pack("<iiiQ", 2, 5, 2, 2) + struct.pack("<8Q", 5, 1, 2, 3, 6, 1, 3, 4)
+    b += b"\n$EndElements\n"
+    open(path, "wb").write(bytes(b))
+
+
+@pytest.mark.parametrize("fmt", ["v22", "v41b"])
+def test_other_msh_formats(tmp_path, fmt):
+    p = str(tmp_path / "m.msh")
+    _two_triangles_v22(p) if fmt == "v22" else _two_triangles_v41_binary(p)
+    m = dgk.read_mesh(p, conf("Wall=Reflecting\n"))
+    assert (m.dim, m.order, m.K) == (2, 1, 2)
+    assert m.node_tags.tolist() == [1, 2, 3, 1, 3, 4]
+    assert m.bc_groups[0][0] == 7 and m.bc_groups[0][1] == 1 and m.bc_groups[0][2].tolist() == [1, 2, 3, 4]
+    assert (m.node_bc == 1).all()
+
+
+def test_truncated_mesh_is_an_error(tmp_path, reference_dir):
+    src = open(os.path.join(reference_dir, "doc/2d/square.msh")).read()
+    p = tmp_path / "t.msh"
+    p.write_text(src[: len(src) // 2])
+    with pytest.raises(RuntimeError):
+        dgk.read_mesh(str(p), conf())
+
+
+# ----------------------------------------------------------------------------- generators
+def test_generator_sizes_and_orientation():
+    m = dgk.gen_rect(40, 40, 1.0, 1.0, 3, [1, 1, 1, 1], "1:Wall", conf("Wall=Reflecting\n"))
+    assert (m.K, m.Np) == (3200, 10)
+    X = m.node_xyz.reshape(m.K, m.Np, 3)
+    a, b = X[:, 1] - X[:, 0], X[:, 2] - X[:, 0]
+    assert (a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0] > 0).all()
+    m = dgk.gen_box(4, 3, 2, 1.0, 1.0, 1.0, 3, 0.1, 2, [1] * 6, "1:Abs", conf("Abs=Absorbing\n"))
+    assert (m.K, m.Np) == (6 * 24, 20)
+    X = m.node_xyz.reshape(m.K, m.Np, 3)
+    J = np.stack([X[:, k] - X[:, 0] for k in (1, 2, 3)], axis=-1)
+    assert (np.linalg.det(J) > 0).all()
+    # shared high-order nodes carry one tag and one position
+    tags, idx = np.unique(m.node_tags, return_index=True)
+    pos = {t: m.node_xyz[i] for t, i in zip(tags, idx)}
+    for t, x in zip(m.node_tags, m.node_xyz):
+        assert np.array_equal(pos[t], x)
+
+
+def test_room_block_count():
+    block = ((4, 4, 0), (12, 12, 4))
+    m = dgk.gen_box(20, 15, 8, 5.0, 3.75, 2.0, 1, 0.0, 0, [1, 1, 1, 1, 1, 2], "1:Walls;2:Ceiling",
+                    conf("Walls=Reflecting\nCeiling=Absorbing\n"), block=block, block_tag=1)
+    assert m.K == 6 * (20 * 15 * 8 - 8 * 8 * 4)
+
+
+def test_first_node_bc_rule_matches_oracle():
+    """Corner nodes belong to both groups: the later (higher-tag) group wins, and a face's BC
+    is read at its first node (src/Mesh.cpp:364-384)."""
+    for order in (1, 2):
+        m = dgk.gen_rect(3, 3, 1.0, 1.0, order, [1, 2, 1, 2], "1:Wall;2:Abs",
+                         conf("Wall=Reflecting\nAbs=Absorbing\n"))
+        o = ob.Oracle(m, 340.0, 1.225)
+        c = o.connectivity()
+        r = dgk.ref_element(2, order)
+        got = []
+        for e in range(m.K):
+            for j in range(3):
+                f = c["elFIds"][e, j]
+                if c["fBC"][f] >= 0:
+                    got.append((int(c["fBC"][f]), int(m.node_bc[e * m.Np + r["fmask"][j, 0]])))
+        assert got and all(a == b for a, b in got)
+        assert {a for a, _ in got} == {0, 1}
+
+
+# ----------------------------------------------------------------------------- CLI
+def test_cli_wrong_argc_returns_e2big():
+    exe = os.path.join(dgk.PKG_DIR, "dgalerkin")
+    if not os.path.exists(exe):
+        pytest.skip("dgalerkin not built")
+    r = subprocess.run([exe], capture_output=True)
+    assert r.returncode == 7  # E2BIG

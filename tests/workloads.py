@@ -1,0 +1,65 @@
+"""Workload definitions shared by tests and bench.py (meshes, physics, initial states).
+
+Sizes follow the benchmark configurations: a structured 2D square at order 3, a jittered
+unit cube with a Kuhn tetrahedral split at order 3, the 20x15x8 m room with a block
+obstacle at order 4, plus small variants of each for parity tests.
+"""
+import numpy as np
+
+import dgk
+
+BASE_CONF = ("timeStart=0\ntimeEnd=1\ntimeStep=1\ntimeRate=1\nelementType=Lagrange\ntimeIntMethod=Runge-Kutta\n"
+             "numThreads=1\nv0_x=0\nv0_y=0\nv0_z=0\nrho0=1.225\nc0=340\n")
+
+
+def conf(extra=""):
+    return BASE_CONF + extra
+
+
+def gaussian(mesh, center, size, amp=1.0):
+    x = mesh.node_xyz
+    u = np.zeros((4, mesh.K * mesh.Np))
+    r2 = ((x[:, 0] - center[0]) ** 2 + (x[:, 1] - center[1]) ** 2 + (x[:, 2] - center[2]) ** 2)
+    u[0] = amp * np.exp(-r2 / size)
+    return u
+
+
+def square_2d(n=40, order=3):
+    """[0,1]^2, n x n cells, 2 CCW triangles each, one reflecting group 'Wall'."""
+    m = dgk.gen_rect(n, n, 1.0, 1.0, order, [1, 1, 1, 1], "1:Wall", conf("Wall=Reflecting\n"))
+    phys = dict(c0=340.0, rho0=1.225, v0=(0.0, 0.0, 0.0))
+    h = 1.0 / n
+    dt = 0.1 * h / 340.0
+    u0 = gaussian(m, (0.5, 0.5, 0.0), 0.005)
+    return m, phys, dt, u0
+
+
+def cube_3d(n=32, order=3, jitter=0.1, seed=2):
+    """[0,1]^3, n^3 cells Kuhn-split (6n^3 tets), interior nodes jittered, all faces absorbing."""
+    m = dgk.gen_box(n, n, n, 1.0, 1.0, 1.0, order, jitter, seed, [1] * 6, "1:Abs", conf("Abs=Absorbing\n"))
+    phys = dict(c0=340.0, rho0=1.225, v0=(50.0, 0.0, 0.0))
+    rng = np.random.Generator(np.random.MT19937(3))
+    center = 0.3 + 0.4 * rng.random(3)
+    h = 1.0 / n
+    hmin = (1 - 2 * jitter) * h
+    dt = 0.1 * hmin / (340.0 + 50.0)
+    u0 = gaussian(m, center, 2 * 0.04 ** 2)
+    return m, phys, dt, u0
+
+
+def room_3d(h=0.25, order=4, seed=4):
+    """20x15x8 m room, Kuhn tets of size h, a 2x2x1 m block on the floor; walls reflecting, ceiling absorbing."""
+    nx, ny, nz = int(round(20 / h)), int(round(15 / h)), int(round(8 / h))
+    rng = np.random.Generator(np.random.MT19937(seed))
+    bx, by = int(2 / h), int(2 / h)
+    i0 = int(rng.integers(0, nx - bx))
+    j0 = int(rng.integers(0, ny - by))
+    block = ((i0, j0, 0), (i0 + bx, j0 + by, int(round(1 / h))))
+    m = dgk.gen_box(nx, ny, nz, 20.0, 15.0, 8.0, order, 0.0, 0, [1, 1, 1, 1, 1, 2], "1:Walls;2:Ceiling",
+                    conf("Walls=Reflecting\nCeiling=Absorbing\nc0=343\n"), block=block, block_tag=1)
+    phys = dict(c0=343.0, rho0=1.225, v0=(0.0, 0.0, 0.0))
+    sigma = 343.0 / (2 * np.pi * 500.0)
+    center = (4.0 + 12.0 * rng.random(), 3.0 + 9.0 * rng.random(), 1.5 + 5.0 * rng.random())
+    dt = 0.1 * h / 343.0
+    u0 = gaussian(m, center, 2 * sigma ** 2)
+    return m, phys, dt, u0
