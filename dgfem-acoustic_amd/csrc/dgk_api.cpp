@@ -90,6 +90,30 @@ void upload_ops(Ctx& c, const RefElem& R, const std::vector<int>& vmap) {
   DGK_HIP(hipMemcpy(c.ops.fmask, fm.data(), fm.size() * sizeof(int), hipMemcpyHostToDevice));
   c.ops.vmap = (int*)c.alloc(vmap.size() * sizeof(int));
   DGK_HIP(hipMemcpy(c.ops.vmap, vmap.data(), vmap.size() * sizeof(int), hipMemcpyHostToDevice));
+  // MFMA A fragments: k-step s < d*NG covers W_r (r = s % d) at node 4*(s/d) + g for lane group g;
+  // later k-steps cover face entry 4*(s - d*NG) + g of -LIFT. Rows beyond Np are zero.
+  int KS = 0, MB = 0;
+  mfma_shape(R.dim, R.p, &KS, &MB);
+  const int Np = R.Np, NFF = R.Nf * R.Nfp, NG = (Np + 3) / 4, d = R.dim;
+  std::vector<T> A((size_t)MB * KS * 64, (T)0);
+  for (int mb = 0; mb < MB; ++mb)
+    for (int s = 0; s < KS; ++s)
+      for (int l = 0; l < 64; ++l) {
+        int i = 16 * mb + (l % 16), g = l / 16;
+        double v = 0;
+        if (i < Np) {
+          if (s < d * NG) {
+            int r = s % d, n = 4 * (s / d) + g;
+            if (n < Np) v = R.W[(r * Np + i) * Np + n];
+          } else {
+            int jk = 4 * (s - d * NG) + g;
+            if (jk < NFF) v = -R.LIFT[i * NFF + jk];
+          }
+        }
+        A[((size_t)mb * KS + s) * 64 + l] = (T)v;
+      }
+  c.ops.Amf = c.alloc(A.size() * sizeof(T));
+  DGK_HIP(hipMemcpy(c.ops.Amf, A.data(), A.size() * sizeof(T), hipMemcpyHostToDevice));
 }
 
 // vmap[(jn*nperm + perm)*Nfp + k]: neighbour element node matching own face node k when the
@@ -134,6 +158,8 @@ void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* 
   c.sign_mode = o ? o->flux_sign : DGK_SIGN_REFERENCE;
   c.device = o ? o->device : 0;
   c.variant = o ? o->variant : 0;
+  if (c.variant == 0) c.variant = kVariantMfma;
+  if (c.variant != kVariantLds && c.variant != kVariantMfma) throw InvalidArg("unknown stage-kernel variant");
   if (c.precision != DGK_FP64 && c.precision != DGK_FP32) throw InvalidArg("precision must be DGK_FP64 or DGK_FP32");
   if (c.sign_mode != DGK_SIGN_REFERENCE && c.sign_mode != DGK_SIGN_STABLE) throw InvalidArg("bad flux_sign");
   c.tsize = c.precision == DGK_FP64 ? 8 : 4;
@@ -252,6 +278,7 @@ StageArgs base_args(Ctx& c) {
   a.LIFT = c.ops.LIFT;
   a.fmask = c.ops.fmask;
   a.vmap = c.ops.vmap;
+  a.Amf = c.ops.Amf;
   for (int j = 0; j < kMaxNF; ++j) {
     for (int x = 0; x < 3; ++x) a.refn[j][x] = c.refn[j][x];
     a.fsc[j] = c.fsc[j];

@@ -32,6 +32,9 @@ struct InvalidArg : std::runtime_error {
 };
 
 constexpr int kMaxNF = 4;
+// stage-kernel variants (dgk_opts.variant; 0 picks the default)
+constexpr int kVariantLds = 1;   // thread per element node, operators read from memory
+constexpr int kVariantMfma = 2;  // 16x16x4 MFMA contraction, one wave per 4 elements
 constexpr int kMaxNFP = 21;
 constexpr int kMaxPerm = 6;
 
@@ -41,6 +44,7 @@ struct DevOps {
   void* LIFT = nullptr;  // [Np][Nf*Nfp]
   int* fmask = nullptr;  // [Nf][Nfp]
   int* vmap = nullptr;   // [Nf*nperm][Nfp] neighbour element node of own face node k
+  void* Amf = nullptr;   // [MB][KS][64] MFMA A fragments of [W_1..W_d | -LIFT] in k-step order
 };
 
 // Per-stage kernel arguments (precision-agnostic pointers).
@@ -57,6 +61,7 @@ struct StageArgs {
   const void* LIFT;
   const int* fmask;
   const int* vmap;
+  const void* Amf;
   double refn[kMaxNF][3];  // outward reference-face normals
   double fsc[kMaxNF];      // reference face measure / face reference-element measure
   double dt, c0, rho0, v0[3];
@@ -118,6 +123,7 @@ bool check_finite(Ctx& c);
 void convert_state(Ctx& c, const double* src_dev, void* dst_dev, int64_t n);       // FP64 -> state type
 void convert_state_back(Ctx& c, const void* src_dev, double* dst_dev, int64_t n);  // state type -> FP64
 const char* variant_name(int v);
+void mfma_shape(int dim, int p, int* KS, int* MB);
 
 }  // namespace dgk
 

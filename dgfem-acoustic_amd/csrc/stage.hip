@@ -24,6 +24,7 @@
 
 #include "dgk_internal.hpp"
 
+
 namespace dgk {
 
 template <int D, int P>
@@ -184,6 +185,266 @@ __global__ __launch_bounds__(kBlock) void k_stage_lds(StageArgs a) {
   }
 }
 
+
+// ---------------------------------------------------------------------------------------------
+// MFMA variant. One workgroup (4 waves) owns a tile of 16 elements; each wave owns 4 of them and
+// computes R (Np x 16 columns = 4 elements x 4 fields) = Wcat (Np x KD) * X (KD x 16) with
+// 16x16x4 MFMAs, where Wcat = [W_1 .. W_d | -LIFT] and X stacks the contravariant nodal fluxes
+// and the scaled nodal face fluxes. The K dimension is permuted so that lane group g (lanes
+// 16g..16g+15) always supplies nodes n = 4*m + g: at volume k-step s = m*d + r the lane builds
+// Ft_r at node 4m+g from the LDS state tile with per-lane coefficients, at face k-step s the lane
+// reads face entry 4*(s - d*NG) + g of its column from LDS. The matching A fragments are laid out
+// on the host as [m-block][k-step][lane] so that every fragment is one coalesced 64-lane load.
+// MFMA register layouts: A lane l <- A[l%16][l/16], B lane l <- B[l/16][l%16]; the
+// accumulator row of register j in lane l is given by d_row<T> (column l%16).
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double V __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ V mma(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<float> {
+  typedef float V __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ V mma(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <int D, int P>
+struct MfmaShape {
+  using S = Sz<D, P>;
+  static constexpr int NP = S::NP, NFF = S::NF * S::NFP;
+  static constexpr int NG = (NP + 3) / 4;     // node groups
+  static constexpr int NFK = (NFF + 3) / 4;   // face k-steps
+  static constexpr int KS = D * NG + NFK;     // k-steps
+  static constexpr int MB = (NP + 15) / 16;   // 16-row blocks
+  static constexpr int TE = 16;               // elements per workgroup tile
+  static constexpr int RB = NFF > NP ? NFF : NP;
+};
+
+// Output mapping of the 16x16x4 accumulator: row of register j in lane l (column is l % 16).
+// v_mfma_f32_16x16x4_f32 puts rows 4*(l/16)..4*(l/16)+3 in one lane; v_mfma_f64_16x16x4_f64
+// interleaves them as l/16 + 4j (both established with an in-bounds probe on gfx950).
+template <typename T>
+__device__ __forceinline__ int d_row(int lane, int j);
+template <>
+__device__ __forceinline__ int d_row<float>(int lane, int j) { return 4 * (lane >> 4) + j; }
+template <>
+__device__ __forceinline__ int d_row<double>(int lane, int j) { return (lane >> 4) + 4 * j; }
+
+template <typename T, int D, int P>
+__global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
+  using S = Sz<D, P>;
+  using M = MfmaShape<D, P>;
+  constexpr int NP = S::NP, NF = S::NF, NFP = S::NFP, NPERM = S::NPERM, NFF = M::NFF;
+  constexpr int NG = M::NG, NFK = M::NFK, KS = M::KS, MB = M::MB, TE = M::TE, RB = M::RB;
+  constexpr int NE = (4 * TE * NP + kBlock - 1) / kBlock;  // epilogue values per thread
+  constexpr int NI = (TE * NFF + kBlock - 1) / kBlock;     // face items per thread
+  typedef typename Mfma<T>::V V;
+  __shared__ T sU[4][TE * NP];
+  __shared__ T sB[4][TE * RB];
+  __shared__ T sN[TE * NF][4];
+  __shared__ T sG[TE][9];
+  __shared__ int2 sNC[TE * NF];
+  __shared__ int sVmap[NF * NPERM * NFP];
+  __shared__ int sFmask[NF * NFP];
+  constexpr bool kALds = MB * KS * 64 * sizeof(T) <= 32768;  // fragments staged in LDS when they fit
+  __shared__ T sA[kALds ? MB * KS * 64 : 1];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = lane >> 4, col = lane & 15, el = wave * 4 + (col >> 2), q = col & 3;
+  const int64_t K = a.K, KN = a.K * NP;
+  const T* __restrict__ Vin = (const T*)a.Vin;
+  T* __restrict__ Vout = (T*)a.Vout;
+  T* __restrict__ U = (T*)a.U;
+  T* __restrict__ Acc = (T*)a.A;
+  const T* __restrict__ Jinv = (const T*)a.Jinv;
+  const T* __restrict__ Af = (const T*)a.Amf;
+  const T dt = (T)a.dt, c0 = (T)a.c0;
+  const T rc2 = (T)(a.rho0 * a.c0 * a.c0), irho = (T)(1.0 / a.rho0);
+  const T v0[3] = {(T)a.v0[0], (T)a.v0[1], (T)a.v0[2]};
+  const int stage = a.stage;
+  for (int i = tid; i < NF * NPERM * NFP; i += kBlock) sVmap[i] = a.vmap[i];
+  for (int i = tid; i < NF * NFP; i += kBlock) sFmask[i] = a.fmask[i];
+  if (kALds)
+    for (int i = tid; i < MB * KS * 64; i += kBlock) sA[i] = Af[i];
+  const T* __restrict__ pA = (kALds ? (const T*)sA : Af) + lane;
+
+  const int xcd = blockIdx.x % 8, lb = blockIdx.x / 8, bpx = gridDim.x / 8;
+  for (int64_t it = lb; it < a.tiles_per_xcd; it += bpx) {
+    const int64_t tile = (int64_t)xcd * a.tiles_per_xcd + it;
+    if (tile >= a.tiles) break;
+    const int64_t e0 = tile * TE;
+    const int ne = (int)((K - e0) < TE ? (K - e0) : TE);
+    __syncthreads();
+    // phase 1: state tile, geometry, face table, normals; epilogue operands into registers
+    for (int idx = tid; idx < 4 * TE * NP; idx += kBlock) {
+      int qq = idx / (TE * NP), r = idx - qq * (TE * NP);
+      sU[qq][r] = r < ne * NP ? Vin[qq * KN + e0 * NP + r] : (T)0;
+    }
+    for (int idx = tid; idx < TE * 9; idx += kBlock) {
+      int e = idx / 9, c = idx - e * 9;
+      sG[e][c] = e < ne ? Jinv[(e0 + e) * 9 + c] : (T)0;
+    }
+    for (int idx = tid; idx < TE * NF; idx += kBlock) {
+      int e = idx / NF, j = idx - e * NF;
+      int2 nc = make_int2(-1, 0);
+      T n[3] = {0, 0, 0}, len = 1;
+      if (e < ne) {
+        nc = make_int2(a.nbr[(e0 + e) * NF + j], a.code[(e0 + e) * NF + j]);
+        const T* Ge = Jinv + (e0 + e) * 9;
+        for (int x = 0; x < 3; ++x) {
+          T sum = 0;
+          for (int r = 0; r < D; ++r) sum += Ge[r * 3 + x] * (T)a.refn[j][r];
+          n[x] = sum;
+        }
+        len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      }
+      T il = (T)1 / len;
+      sN[idx][0] = n[0] * il;
+      sN[idx][1] = n[1] * il;
+      sN[idx][2] = n[2] * il;
+      sN[idx][3] = len * (T)a.fsc[j];
+      sNC[idx] = nc;
+    }
+    T pu[NE], pa[NE];
+    if (stage == 1 || stage == 2 || stage == 3) {
+#pragma unroll
+      for (int t = 0; t < NE; ++t) {
+        const int idx = tid + t * kBlock;
+        pu[t] = 0;
+        pa[t] = 0;
+        if (idx < 4 * TE * NP) {
+          const int qq = idx / (TE * NP), r = idx - qq * (TE * NP);
+          if (r < ne * NP) {
+            const int64_t id = qq * KN + e0 * NP + r;
+            if (stage != 3) pu[t] = U[id];
+            pa[t] = Acc[id];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // phase 2: nodal normal numerical fluxes scaled by sJ/|detJ| -> sB[q][e*RB + jk]
+#pragma unroll
+    for (int t = 0; t < NI; ++t) {
+      const int idx = tid + t * kBlock;
+      if (idx >= TE * NFF) break;
+      const int e = idx / NFF, jk = idx - e * NFF, j = jk / NFP, k = jk - j * NFP;
+      const int ej = e * NF + j;
+      T fl[4] = {0, 0, 0, 0};
+      const int2 nc = sNC[ej];
+      if (e < ne) {
+        const T n[3] = {sN[ej][0], sN[ej][1], sN[ej][2]};
+        const T fs = sN[ej][3];
+        const int on = sFmask[jk];
+        const T qm[4] = {sU[0][e * NP + on], sU[1][e * NP + on], sU[2][e * NP + on], sU[3][e * NP + on]};
+        const T vn = v0[0] * n[0] + v0[1] * n[1] + v0[2] * n[2];
+        const T unm = qm[1] * n[0] + qm[2] * n[1] + qm[3] * n[2];
+        if (nc.x >= 0) {
+          const int node = sVmap[(nc.y & 0xffff) * NFP + k];
+          const T hc = (T)0.5 * ((nc.y >> 16) & 1 ? -c0 : c0);
+          const int64_t pi = (int64_t)nc.x * NP + node;
+          const T qp[4] = {Vin[pi], Vin[KN + pi], Vin[2 * KN + pi], Vin[3 * KN + pi]};
+          const T unp = qp[1] * n[0] + qp[2] * n[1] + qp[3] * n[2];
+          fl[0] = (T)0.5 * ((vn * qm[0] + rc2 * unm) + (vn * qp[0] + rc2 * unp)) + hc * (qm[0] - qp[0]);
+          for (int c = 0; c < 3; ++c)
+            fl[1 + c] = (T)0.5 * ((vn * qm[1 + c] + qm[0] * irho * n[c]) + (vn * qp[1 + c] + qp[0] * irho * n[c])) +
+                        hc * (qm[1 + c] - qp[1 + c]);
+        } else if (nc.x == -2) {
+          const T gv[3] = {qm[1] - unm * n[0], qm[2] - unm * n[1], qm[3] - unm * n[2]};
+          const T ung = gv[0] * n[0] + gv[1] * n[1] + gv[2] * n[2];
+          fl[0] = vn * qm[0] + rc2 * ung;
+          for (int c = 0; c < 3; ++c) fl[1 + c] = vn * gv[c] + qm[0] * irho * n[c];
+        } else {
+          fl[0] = (T)0.25 * (c0 * qm[0] + rc2 * unm);
+          const T tt = (T)0.25 * (qm[0] * irho + c0 * unm);
+          for (int c = 0; c < 3; ++c) fl[1 + c] = tt * n[c];
+        }
+        for (int qq = 0; qq < 4; ++qq) fl[qq] *= fs;
+      }
+      for (int qq = 0; qq < 4; ++qq) sB[qq][e * RB + jk] = fl[qq];
+    }
+    __syncthreads();
+    // phase 3: per-lane coefficients Ft_r = alpha_r p + sum_x beta_rx v_x, then the MFMA chain
+    T alpha[D], beta[D][3];
+    {
+      const T* G = sG[el];
+      for (int r = 0; r < D; ++r) {
+        const T at = G[r * 3 + 0] * v0[0] + G[r * 3 + 1] * v0[1] + G[r * 3 + 2] * v0[2];
+        if (q == 0) {
+          alpha[r] = at;
+          for (int x = 0; x < 3; ++x) beta[r][x] = rc2 * G[r * 3 + x];
+        } else {
+          alpha[r] = G[r * 3 + (q - 1)] * irho;
+          for (int x = 0; x < 3; ++x) beta[r][x] = (x == q - 1) ? at : (T)0;
+        }
+      }
+    }
+    V acc[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) acc[mb] = V{0, 0, 0, 0};
+#pragma unroll
+    for (int m = 0; m < NG; ++m) {
+      const int n = 4 * m + g;
+      T p = 0, vx = 0, vy = 0, vz = 0;
+      if (n < NP) {
+        p = sU[0][el * NP + n];
+        vx = sU[1][el * NP + n];
+        vy = sU[2][el * NP + n];
+        vz = sU[3][el * NP + n];
+      }
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        const T b = alpha[r] * p + beta[r][0] * vx + beta[r][1] * vy + beta[r][2] * vz;
+        const int s = m * D + r;
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) acc[mb] = Mfma<T>::mma(pA[(mb * KS + s) * 64], b, acc[mb]);
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < NFK; ++f) {
+      const int jk = 4 * f + g;
+      const T b = jk < NFF ? sB[q][el * RB + jk] : (T)0;
+      const int s = D * NG + f;
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) acc[mb] = Mfma<T>::mma(pA[(mb * KS + s) * 64], b, acc[mb]);
+    }
+    __syncthreads();  // face fluxes consumed; sB now receives the results
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int i = 16 * mb + d_row<T>(lane, j);
+        if (i < NP) sB[q][el * RB + i] = acc[mb][j];
+      }
+    __syncthreads();
+    // phase 4: coalesced time-integration epilogue
+#pragma unroll
+    for (int t = 0; t < NE; ++t) {
+      const int idx = tid + t * kBlock;
+      if (idx >= 4 * TE * NP) break;
+      const int qq = idx / (TE * NP), r = idx - qq * (TE * NP), e = r / NP, i = r - e * NP;
+      if (e >= ne) continue;
+      const T kq = dt * sB[qq][e * RB + i];
+      const int64_t id = qq * KN + e0 * NP + r;
+      const T s6 = (T)(1.0 / 6.0), s3 = (T)(1.0 / 3.0), s2 = (T)0.5;
+      switch (stage) {
+        case 0: { const T u = sU[qq][r]; Acc[id] = u + kq * s6; Vout[id] = u + kq * s2; } break;
+        case 1: Acc[id] = pa[t] + kq * s3; Vout[id] = pu[t] + kq * s2; break;
+        case 2: Acc[id] = pa[t] + kq * s3; Vout[id] = pu[t] + kq; break;
+        case 3: U[id] = pa[t] + kq * s6; break;
+        case 4: Vout[id] = kq; break;
+        default: Vout[id] = sU[qq][r] + kq; break;
+      }
+    }
+  }
+}
+
 template <typename T>
 __global__ void k_source(T* __restrict__ U, const int* __restrict__ idx, int64_t n, T value) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -235,10 +496,32 @@ void launch_lds(StageArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((k_stage_lds<T, D, P>), dim3(8 * bpx), dim3(kBlock), 0, s, a);
 }
 
+template <typename T, int D, int P>
+void launch_mfma(StageArgs& a, hipStream_t s) {
+  constexpr int TE = MfmaShape<D, P>::TE;
+  static int occ = 0;
+  if (!occ) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stage_mfma<T, D, P>, kBlock, 0) != hipSuccess || occ < 1)
+      occ = 1;
+  }
+  a.tiles = (a.K + TE - 1) / TE;
+  a.tiles_per_xcd = (int)((a.tiles + 7) / 8);
+  int64_t per_xcd_cap = (int64_t)(num_cus() / 8) * occ;
+  int bpx = (int)std::min<int64_t>(a.tiles_per_xcd, per_xcd_cap);
+  if (bpx < 1) bpx = 1;
+  hipLaunchKernelGGL((k_stage_mfma<T, D, P>), dim3(8 * bpx), dim3(kBlock), 0, s, a);
+}
+
+template <typename T, int D, int P>
+void launch_variant(Ctx& c, StageArgs& a, hipStream_t s) {
+  if (c.variant == kVariantLds) launch_lds<T, D, P>(a, s);
+  else launch_mfma<T, D, P>(a, s);
+}
+
 template <typename T>
 void dispatch(Ctx& c, StageArgs& a, hipStream_t s) {
 #define DGK_CASE(D, P) \
-  if (c.dim == D && c.p == P) return launch_lds<T, D, P>(a, s);
+  if (c.dim == D && c.p == P) return launch_variant<T, D, P>(c, a, s);
   DGK_CASE(1, 1) DGK_CASE(1, 2) DGK_CASE(1, 3) DGK_CASE(1, 4) DGK_CASE(1, 5)
   DGK_CASE(2, 1) DGK_CASE(2, 2) DGK_CASE(2, 3) DGK_CASE(2, 4) DGK_CASE(2, 5)
   DGK_CASE(3, 1) DGK_CASE(3, 2) DGK_CASE(3, 3) DGK_CASE(3, 4) DGK_CASE(3, 5)
@@ -298,9 +581,20 @@ void convert_state_back(Ctx& c, const void* src, double* dst, int64_t n) {
 
 const char* variant_name(int v) {
   switch (v) {
-    case 0: return "lds-tile";
+    case kVariantLds: return "lds-tile";
+    case kVariantMfma: return "mfma16";
     default: return "unknown";
   }
+}
+
+void mfma_shape(int dim, int p, int* KS, int* MB) {
+#define DGK_SHAPE(D, P) \
+  if (dim == D && p == P) { *KS = MfmaShape<D, P>::KS; *MB = MfmaShape<D, P>::MB; return; }
+  DGK_SHAPE(1, 1) DGK_SHAPE(1, 2) DGK_SHAPE(1, 3) DGK_SHAPE(1, 4) DGK_SHAPE(1, 5)
+  DGK_SHAPE(2, 1) DGK_SHAPE(2, 2) DGK_SHAPE(2, 3) DGK_SHAPE(2, 4) DGK_SHAPE(2, 5)
+  DGK_SHAPE(3, 1) DGK_SHAPE(3, 2) DGK_SHAPE(3, 3) DGK_SHAPE(3, 4) DGK_SHAPE(3, 5)
+#undef DGK_SHAPE
+  *KS = 0; *MB = 0;
 }
 
 }  // namespace dgk

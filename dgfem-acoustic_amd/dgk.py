@@ -151,6 +151,7 @@ def gen_line(n, L, order, side_tags, names, config_text):
 # ----------------------------------------------------------------------------- device library
 FP64, FP32 = 0, 1
 SIGN_REFERENCE, SIGN_STABLE = 0, 1
+VARIANT_DEFAULT, VARIANT_LDS, VARIANT_MFMA = 0, 1, 2
 
 
 class _Mesh(ctypes.Structure):

@@ -30,14 +30,18 @@ def mixed_mesh(dim, p, n=None):
 CASES = [(d, p) for d in (1, 2, 3) for p in (1, 2, 3, 4, 5)]
 
 
+VARIANTS = [dgk.VARIANT_MFMA, dgk.VARIANT_LDS]
+
+
 @pytest.mark.parametrize("dim,p", CASES)
 @pytest.mark.parametrize("sign", [dgk.SIGN_REFERENCE, dgk.SIGN_STABLE])
-def test_rhs_and_rk4_fp64(dim, p, sign):
-    print(f"case dim={dim} p={p} sign={sign}", flush=True)
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_rhs_and_rk4_fp64(dim, p, sign, variant):
+    print(f"case dim={dim} p={p} sign={sign} variant={variant}", flush=True)
     m = mixed_mesh(dim, p)
     v0 = (40.0, -20.0, 10.0)
     o = ob.Oracle(m, 340.0, 1.225, v0=v0, sign_mode=sign)
-    s = dgk.Solver(m, 340.0, 1.225, v0=v0, flux_sign=sign)
+    s = dgk.Solver(m, 340.0, 1.225, v0=v0, flux_sign=sign, variant=variant)
     u = np.random.default_rng(11).standard_normal((4, m.K * m.Np))
     assert rel(s.rhs(u, 1e-4), o.rhs(u, 1e-4)) <= 1e-13
     s.set_state(u)
@@ -150,12 +154,24 @@ def test_nonfinite_state_is_reported():
 def test_nonconforming_mesh_is_rejected():
     m = mixed_mesh(2, 2)
     tags = m.node_tags.copy()
-    # give element 0's edge node a tag nobody shares: its face no longer matches its neighbour
-    tags[3] = tags.max() + 1
+    # element 0 is the lower triangle of the first cell; its local edge (1,2) is interior. Give that
+    # edge's node (local node 4 at order 2) a tag nobody shares: the face no longer matches its neighbour
+    tags[4] = tags.max() + 1
     m.node_tags = tags
     with pytest.raises(dgk.DGKError) as ei:
         dgk.Solver(m, 340.0, 1.225)
     assert ei.value.status == 3
+
+
+@pytest.mark.parametrize("dim,p", [(2, 3), (3, 2), (3, 3), (3, 4), (3, 5)])
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_rhs_fp32(dim, p, variant):
+    m = mixed_mesh(dim, p)
+    v0 = (40.0, -20.0, 10.0)
+    o = ob.Oracle(m, 340.0, 1.225, v0=v0, sign_mode=ob.SIGN_STABLE)
+    s = dgk.Solver(m, 340.0, 1.225, v0=v0, flux_sign=dgk.SIGN_STABLE, precision=dgk.FP32, variant=variant)
+    u = np.random.default_rng(5).standard_normal((4, m.K * m.Np))
+    assert rel(s.rhs(u, 1e-4), o.rhs(u, 1e-4)) <= 2e-6
 
 
 @pytest.mark.parametrize("workload", ["square", "cube"])

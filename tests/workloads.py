@@ -1,5 +1,9 @@
 """Workload definitions shared by tests and bench.py (meshes, physics, initial states).
 
+Time steps use a CFL number of 0.1 relative to the DG node spacing, dt = 0.1 h_min /
+(c (2p+1)): with dt = 0.1 h/c the order-3 schemes are unstable under RK4 (2D limit about
+0.06 h/c, 3D about 0.05 h/c for these meshes).
+
 Sizes follow the benchmark configurations: a structured 2D square at order 3, a jittered
 unit cube with a Kuhn tetrahedral split at order 3, the 20x15x8 m room with a block
 obstacle at order 4, plus small variants of each for parity tests.
@@ -16,6 +20,10 @@ def conf(extra=""):
     return BASE_CONF + extra
 
 
+def cfl_dt(hmin, speed, order, cfl=0.1):
+    return cfl * hmin / (speed * (2 * order + 1))
+
+
 def gaussian(mesh, center, size, amp=1.0):
     x = mesh.node_xyz
     u = np.zeros((4, mesh.K * mesh.Np))
@@ -29,7 +37,7 @@ def square_2d(n=40, order=3):
     m = dgk.gen_rect(n, n, 1.0, 1.0, order, [1, 1, 1, 1], "1:Wall", conf("Wall=Reflecting\n"))
     phys = dict(c0=340.0, rho0=1.225, v0=(0.0, 0.0, 0.0))
     h = 1.0 / n
-    dt = 0.1 * h / 340.0
+    dt = cfl_dt(h, 340.0, order)
     u0 = gaussian(m, (0.5, 0.5, 0.0), 0.005)
     return m, phys, dt, u0
 
@@ -42,7 +50,7 @@ def cube_3d(n=32, order=3, jitter=0.1, seed=2):
     center = 0.3 + 0.4 * rng.random(3)
     h = 1.0 / n
     hmin = (1 - 2 * jitter) * h
-    dt = 0.1 * hmin / (340.0 + 50.0)
+    dt = cfl_dt(hmin, 340.0 + 50.0, order)
     u0 = gaussian(m, center, 2 * 0.04 ** 2)
     return m, phys, dt, u0
 
@@ -60,6 +68,6 @@ def room_3d(h=0.25, order=4, seed=4):
     phys = dict(c0=343.0, rho0=1.225, v0=(0.0, 0.0, 0.0))
     sigma = 343.0 / (2 * np.pi * 500.0)
     center = (4.0 + 12.0 * rng.random(), 3.0 + 9.0 * rng.random(), 1.5 + 5.0 * rng.random())
-    dt = 0.1 * h / 343.0
+    dt = cfl_dt(h, 343.0, order)
     u0 = gaussian(m, center, 2 * sigma ** 2)
     return m, phys, dt, u0
