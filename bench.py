@@ -92,15 +92,30 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="fp64,fp32", help="comma list of fp64,fp32")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="first run a plain device copy of MB MiB (reference dispatch for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is in DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copy calibration: {args.copy_calibration} MiB read + {args.copy_calibration} MiB written",
+              file=sys.stderr, flush=True)
+        del src, dst
     m, ph, dt, u0 = wl.cube_3d()
     dof = 4 * m.K * m.Np
     res = {}
-    for name, prec in (("fp64", dgk.FP64), ("fp32", dgk.FP32)):
+    forms = [f for f in (("fp64", dgk.FP64), ("fp32", dgk.FP32)) if f[0] in args.forms.split(",")]
+    for name, prec in forms:
         print(f"[bench] {name}: {args.warmup} warmup + {args.steps} timed RK4 steps, K={m.K}", file=sys.stderr,
               flush=True)
         s, wall, kern_ms, state, finite, nsteps_total = run_form(m, ph, dt, u0, prec, args.steps, args.warmup)
@@ -116,6 +131,10 @@ def main():
         s.close()
         print(f"[bench] {name}: {rate:.4g} DOF-stage/s wall, stage kernel {kern_ms * 1e3:.1f} us "
               f"({ach_bw / 1e9:.0f} GB/s algorithmic)", file=sys.stderr, flush=True)
+    if len(res) < 2:
+        for k, r in res.items():
+            print(json.dumps({"form": k, "rate": r["rate"], "kernel_avg_us": r["kern_ms"] * 1e3}))
+        return
     g64, g32 = res["fp64"]["state"], res["fp32"]["state"]
     names = ["p", "vx", "vy", "vz"]
     errs = {names[q]: float(np.linalg.norm(g32[q] - g64[q]) / max(np.linalg.norm(g64[q]), 1e-300)) for q in range(4)}

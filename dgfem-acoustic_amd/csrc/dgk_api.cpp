@@ -150,6 +150,8 @@ void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* 
   if (m->dim < 1 || m->dim > 3) throw Unsupported("element dimension must be 1, 2 or 3");
   if (m->order < 1 || m->order > 5) throw Unsupported("element order must be 1..5");
   if (m->num_elements <= 0 || !m->node_xyz || !m->node_tags) throw InvalidArg("empty mesh");
+  if (4 * (m->num_elements + kTilePad) * num_nodes(m->dim, m->order) >= ((int64_t)1 << 31))
+    throw Unsupported("4*K*Np must stay below 2^31 (32-bit device indexing)");
   if (!(ph->c0 > 0) || !(ph->rho0 > 0)) throw InvalidArg("c0 and rho0 must be positive");
   c.dim = m->dim;
   c.p = m->order;
@@ -221,19 +223,27 @@ void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* 
 
   DGK_HIP(hipSetDevice(c.device));
   DGK_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-  const int64_t n = 4 * K * Np;
+  // device arrays cover Kpad elements; padding elements have zero state and geometry and an
+  // absorbing "face" with zero normal, so kernels can treat every tile as full
+  c.Kpad = (K + kTilePad - 1) / kTilePad * kTilePad;
+  c.PS = c.Kpad * Np;
+  const int64_t n = 4 * c.PS, KP = c.Kpad;
   c.U = c.alloc(n * c.tsize);
   c.A = c.alloc(n * c.tsize);
   c.V0 = c.alloc(n * c.tsize);
   c.V1 = c.alloc(n * c.tsize);
-  c.Jinv = c.alloc(K * 9 * c.tsize);
-  c.nbr = (int32_t*)c.alloc(K * c.Nf * sizeof(int32_t));
-  c.code = (int32_t*)c.alloc(K * c.Nf * sizeof(int32_t));
-  c.nface = (int32_t*)c.alloc(K * c.Nf * sizeof(int32_t));
-  c.perm = (int32_t*)c.alloc(K * c.Nf * sizeof(int32_t));
+  c.Jinv = c.alloc(KP * 9 * c.tsize);
+  c.fgeom = c.alloc(KP * c.Nf * 4 * c.tsize);
+  c.nbr = (int32_t*)c.alloc(KP * c.Nf * sizeof(int32_t));
+  c.code = (int32_t*)c.alloc(KP * c.Nf * sizeof(int32_t));
+  c.nface = (int32_t*)c.alloc(KP * c.Nf * sizeof(int32_t));
+  c.perm = (int32_t*)c.alloc(KP * c.Nf * sizeof(int32_t));
   c.flag = (int*)c.alloc(sizeof(int));
-  DGK_HIP(hipMemsetAsync(c.U, 0, n * c.tsize, c.stream));
-  DGK_HIP(hipMemsetAsync(c.A, 0, n * c.tsize, c.stream));
+  for (void* p : {c.U, c.A, c.V0, c.V1}) DGK_HIP(hipMemsetAsync(p, 0, n * c.tsize, c.stream));
+  DGK_HIP(hipMemsetAsync(c.Jinv, 0, KP * 9 * c.tsize, c.stream));
+  DGK_HIP(hipMemsetAsync(c.fgeom, 0, KP * c.Nf * 4 * c.tsize, c.stream));
+  DGK_HIP(hipMemsetAsync(c.nbr, 0xff, KP * c.Nf * sizeof(int32_t), c.stream));
+  DGK_HIP(hipMemsetAsync(c.code, 0, KP * c.Nf * sizeof(int32_t), c.stream));
   double* vx = nullptr;
   DGK_HIP(hipMalloc(&vx, vxyz.size() * sizeof(double)));
   try {
@@ -269,9 +279,12 @@ void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* 
 StageArgs base_args(Ctx& c) {
   StageArgs a{};
   a.K = c.K;
+  a.Kpad = c.Kpad;
+  a.PS = c.PS;
   a.U = c.U;
   a.A = c.A;
   a.Jinv = c.Jinv;
+  a.fgeom = c.fgeom;
   a.nbr = c.nbr;
   a.code = c.code;
   a.W = c.ops.W;
@@ -372,8 +385,34 @@ void euler_impl(Ctx& c, int nsteps, double dt, const double* ts) {
 }
 
 double* staging(Ctx& c) {
-  if (!c.stage_d) c.stage_d = (double*)c.alloc(4 * c.K * c.Np * sizeof(double));
+  if (!c.stage_d) {
+    c.stage_d = (double*)c.alloc(4 * c.PS * sizeof(double));
+    DGK_HIP(hipMemsetAsync(c.stage_d, 0, 4 * c.PS * sizeof(double), c.stream));
+  }
   return c.stage_d;
+}
+
+// Host u[4][K*Np] (FP64) <-> device planes with stride PS in the state precision.
+void upload_planes(Ctx& c, const double* u, void* dst) {
+  const size_t row = (size_t)c.K * c.Np * sizeof(double);
+  if (c.precision == DGK_FP64) {
+    DGK_HIP(hipMemcpy2DAsync(dst, c.PS * sizeof(double), u, row, row, 4, hipMemcpyHostToDevice, c.stream));
+    return;
+  }
+  double* s = staging(c);
+  DGK_HIP(hipMemcpy2DAsync(s, c.PS * sizeof(double), u, row, row, 4, hipMemcpyHostToDevice, c.stream));
+  convert_state(c, s, dst, 4 * c.PS);
+}
+
+void download_planes(Ctx& c, const void* src, double* u) {
+  const size_t row = (size_t)c.K * c.Np * sizeof(double);
+  if (c.precision == DGK_FP64) {
+    DGK_HIP(hipMemcpy2DAsync(u, row, src, c.PS * sizeof(double), row, 4, hipMemcpyDeviceToHost, c.stream));
+    return;
+  }
+  double* s = staging(c);
+  convert_state_back(c, src, s, 4 * c.PS);
+  DGK_HIP(hipMemcpy2DAsync(u, row, s, c.PS * sizeof(double), row, 4, hipMemcpyDeviceToHost, c.stream));
 }
 
 }  // namespace
@@ -417,14 +456,7 @@ dgk_status dgk_info_get(const dgk_ctx* c, dgk_info* info) {
 dgk_status dgk_set_state(dgk_ctx* c, const double* u) {
   if (!c || !u) return fail(c, DGK_ERR_INVALID, "NULL argument");
   return guarded(c, [&] {
-    int64_t n = 4 * c->K * c->Np;
-    if (c->precision == DGK_FP64) {
-      DGK_HIP(hipMemcpyAsync(c->U, u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    } else {
-      double* s = staging(*c);
-      DGK_HIP(hipMemcpyAsync(s, u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-      convert_state(*c, s, c->U, n);
-    }
+    upload_planes(*c, u, c->U);
     DGK_HIP(hipStreamSynchronize(c->stream));
   });
 }
@@ -432,14 +464,7 @@ dgk_status dgk_set_state(dgk_ctx* c, const double* u) {
 dgk_status dgk_get_state(dgk_ctx* c, double* u) {
   if (!c || !u) return fail(c, DGK_ERR_INVALID, "NULL argument");
   return guarded(c, [&] {
-    int64_t n = 4 * c->K * c->Np;
-    if (c->precision == DGK_FP64) {
-      DGK_HIP(hipMemcpyAsync(u, c->U, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    } else {
-      double* s = staging(*c);
-      convert_state_back(*c, c->U, s, n);
-      DGK_HIP(hipMemcpyAsync(u, s, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    }
+    download_planes(*c, c->U, u);
     DGK_HIP(hipStreamSynchronize(c->stream));
   });
 }
@@ -485,18 +510,14 @@ dgk_status dgk_euler(dgk_ctx* c, int nsteps, double dt, const double* t_start) {
 dgk_status dgk_rhs(dgk_ctx* c, const double* u, double* out, double dt) {
   if (!c || !u || !out) return fail(c, DGK_ERR_INVALID, "NULL argument");
   return guarded(c, [&] {
-    int64_t n = 4 * c->K * c->Np;
-    double* s = staging(*c);
-    DGK_HIP(hipMemcpyAsync(s, u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    convert_state(*c, s, c->V1, n);
+    upload_planes(*c, u, c->V1);
     StageArgs a = base_args(*c);
     a.dt = dt;
     a.stage = 4;
     a.Vin = c->V1;
     a.Vout = c->V0;
     launch_stage(*c, a, c->stream);
-    convert_state_back(*c, c->V0, s, n);
-    DGK_HIP(hipMemcpyAsync(out, s, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    download_planes(*c, c->V0, out);
     DGK_HIP(hipStreamSynchronize(c->stream));
   });
 }

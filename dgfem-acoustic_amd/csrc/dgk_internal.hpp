@@ -32,6 +32,7 @@ struct InvalidArg : std::runtime_error {
 };
 
 constexpr int kMaxNF = 4;
+constexpr int kTilePad = 16;  // device arrays hold a multiple of this many elements
 // stage-kernel variants (dgk_opts.variant; 0 picks the default)
 constexpr int kVariantLds = 1;   // thread per element node, operators read from memory
 constexpr int kVariantMfma = 2;  // 16x16x4 MFMA contraction, one wave per 4 elements
@@ -50,11 +51,14 @@ struct DevOps {
 // Per-stage kernel arguments (precision-agnostic pointers).
 struct StageArgs {
   int64_t K;
+  int64_t Kpad;  // element count padded to whole tiles
+  int64_t PS;    // field plane stride in values (Kpad * Np)
   const void* Vin;
   void* Vout;
   void* U;
   void* A;
   const void* Jinv;       // [K][9]
+  const void* fgeom;      // [K][Nf][4]: unit outward normal, sJ/|detJ|
   const int32_t* nbr;     // [K*Nf]
   const int32_t* code;    // [K*Nf]: (nbr_face*nperm + perm) | (penalty sign -1 ? 1<<16 : 0)
   const void* W;
@@ -73,13 +77,14 @@ struct StageArgs {
 struct Ctx {
   int dim = 0, p = 0, Np = 0, Nf = 0, Nfp = 0, nperm = 1;
   int64_t K = 0, F = 0, Fb = 0;
+  int64_t Kpad = 0, PS = 0;  // elements padded to kTilePad, field plane stride (values)
   int precision = DGK_FP64, sign_mode = DGK_SIGN_REFERENCE, device = 0, variant = 0;
   double fc = 1;
   double c0 = 1, rho0 = 1, v0[3] = {0, 0, 0};
   hipStream_t stream = nullptr;
   size_t tsize = 8;  // bytes per state value
   // device buffers
-  void *U = nullptr, *A = nullptr, *V0 = nullptr, *V1 = nullptr, *Jinv = nullptr;
+  void *U = nullptr, *A = nullptr, *V0 = nullptr, *V1 = nullptr, *Jinv = nullptr, *fgeom = nullptr;
   int32_t *nbr = nullptr, *code = nullptr, *nface = nullptr, *perm = nullptr;
   double* stage_d = nullptr;  // FP64 staging for state transfer in FP32 mode
   int* flag = nullptr;

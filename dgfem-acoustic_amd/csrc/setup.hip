@@ -125,9 +125,17 @@ __global__ void k_face_finalize(int64_t K, int D, int NF, int NPERM, const int32
 }
 
 // Affine geometry: Jinv[r][x] = d r / d x (Gmsh reference coordinates), stored 3x3 row-major.
+struct FaceRef {
+  double n[4][3];  // outward reference normals
+  double fsc[4];   // reference face measure / face reference-element measure
+};
+
+// Affine geometry: Jinv[r][x] = d r / d x (Gmsh reference coordinates), stored 3x3 row-major,
+// and per element face the unit outward normal J^-T n_ref / |J^-T n_ref| and the face scale
+// sJ/|detJ| = |J^-T n_ref| * fsc, computed in FP64 and rounded once to T.
 template <typename T>
 __global__ void k_geometry(int64_t K, int D, const double* __restrict__ vxyz, T* __restrict__ Jinv,
-                           int* __restrict__ err) {
+                           T* __restrict__ fgeom, FaceRef fr, int* __restrict__ err) {
   int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= K) return;
   const double* X = vxyz + e * (D + 1) * 3;
@@ -164,6 +172,17 @@ __global__ void k_geometry(int64_t K, int D, const double* __restrict__ vxyz, T*
   }
   for (int r = 0; r < 3; ++r)
     for (int x = 0; x < 3; ++x) Jinv[e * 9 + r * 3 + x] = (T)inv[r][x];
+  for (int j = 0; j <= D; ++j) {
+    double n[3];
+    for (int x = 0; x < 3; ++x) {
+      double s = 0;
+      for (int r = 0; r < D; ++r) s += inv[r][x] * fr.n[j][r];
+      n[x] = s;
+    }
+    double len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for (int x = 0; x < 3; ++x) fgeom[(e * (D + 1) + j) * 4 + x] = (T)(n[x] / len);
+    fgeom[(e * (D + 1) + j) * 4 + 3] = (T)(len * fr.fsc[j]);
+  }
 }
 
 }  // namespace
@@ -234,10 +253,17 @@ void build_geometry(Ctx& c, const double* vxyz_dev) {
   DGK_HIP(hipMemsetAsync(err, 0, sizeof(int), c.stream));
   int bs = 256;
   unsigned nb = (unsigned)((c.K + bs - 1) / bs);
+  FaceRef fr{};
+  for (int j = 0; j < 4; ++j) {
+    for (int x = 0; x < 3; ++x) fr.n[j][x] = c.refn[j][x];
+    fr.fsc[j] = c.fsc[j];
+  }
   if (c.precision == DGK_FP64)
-    hipLaunchKernelGGL(k_geometry<double>, dim3(nb), dim3(bs), 0, c.stream, c.K, c.dim, vxyz_dev, (double*)c.Jinv, err);
+    hipLaunchKernelGGL(k_geometry<double>, dim3(nb), dim3(bs), 0, c.stream, c.K, c.dim, vxyz_dev, (double*)c.Jinv,
+                       (double*)c.fgeom, fr, err);
   else
-    hipLaunchKernelGGL(k_geometry<float>, dim3(nb), dim3(bs), 0, c.stream, c.K, c.dim, vxyz_dev, (float*)c.Jinv, err);
+    hipLaunchKernelGGL(k_geometry<float>, dim3(nb), dim3(bs), 0, c.stream, c.K, c.dim, vxyz_dev, (float*)c.Jinv,
+                       (float*)c.fgeom, fr, err);
   int herr = 0;
   hipError_t e1 = hipGetLastError();
   hipError_t e2 = hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, c.stream);

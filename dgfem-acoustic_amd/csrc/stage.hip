@@ -52,7 +52,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_lds(StageArgs a) {
   __shared__ T sN[TE][NF][4];
 
   const int tid = threadIdx.x;
-  const int64_t K = a.K, KN = a.K * NP;
+  const int64_t K = a.K, KN = a.PS;  // KN: plane stride
   const T* __restrict__ Vin = (const T*)a.Vin;
   T* __restrict__ Vout = (T*)a.Vout;
   T* __restrict__ U = (T*)a.U;
@@ -242,27 +242,33 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
   using M = MfmaShape<D, P>;
   constexpr int NP = S::NP, NF = S::NF, NFP = S::NFP, NPERM = S::NPERM, NFF = M::NFF;
   constexpr int NG = M::NG, NFK = M::NFK, KS = M::KS, MB = M::MB, TE = M::TE, RB = M::RB;
-  constexpr int NE = (4 * TE * NP + kBlock - 1) / kBlock;  // epilogue values per thread
-  constexpr int NI = (TE * NFF + kBlock - 1) / kBlock;     // face items per thread
+  constexpr int VW = 16 / sizeof(T);                        // values per 16-byte vector
+  typedef T VT __attribute__((ext_vector_type(VW)));
+  constexpr int PV = TE * NP / VW;                          // vectors per field plane of a tile
+  constexpr int NE = (4 * PV + kBlock - 1) / kBlock;        // state vectors per thread
+  constexpr int NI = (TE * NFF + kBlock - 1) / kBlock;      // face items per thread
+  constexpr int NGEO = (TE * 9 + kBlock - 1) / kBlock;      // geometry values per thread
+  constexpr int NFC = (TE * NF + kBlock - 1) / kBlock;      // element faces per thread
   typedef typename Mfma<T>::V V;
-  __shared__ T sU[4][TE * NP];
-  __shared__ T sB[4][TE * RB];
+  __shared__ __attribute__((aligned(16))) T sU[4][TE * NP];
+  __shared__ __attribute__((aligned(16))) T sB[4][TE * RB];  // face fluxes, then results as [4][TE*NP]
   __shared__ T sN[TE * NF][4];
   __shared__ T sG[TE][9];
-  __shared__ int2 sNC[TE * NF];
   __shared__ int sVmap[NF * NPERM * NFP];
   __shared__ int sFmask[NF * NFP];
   constexpr bool kALds = MB * KS * 64 * sizeof(T) <= 32768;  // fragments staged in LDS when they fit
   __shared__ T sA[kALds ? MB * KS * 64 : 1];
+  T* const sR = &sB[0][0];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, col = lane & 15, el = wave * 4 + (col >> 2), q = col & 3;
-  const int64_t K = a.K, KN = a.K * NP;
+  const int PS = (int)a.PS;  // plane stride: Kpad*Np, 4*PS < 2^31 is checked at context creation
   const T* __restrict__ Vin = (const T*)a.Vin;
   T* __restrict__ Vout = (T*)a.Vout;
   T* __restrict__ U = (T*)a.U;
   T* __restrict__ Acc = (T*)a.A;
   const T* __restrict__ Jinv = (const T*)a.Jinv;
+  const T* __restrict__ Fg = (const T*)a.fgeom;
   const T* __restrict__ Af = (const T*)a.Amf;
   const T dt = (T)a.dt, c0 = (T)a.c0;
   const T rc2 = (T)(a.rho0 * a.c0 * a.c0), irho = (T)(1.0 / a.rho0);
@@ -274,101 +280,140 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
     for (int i = tid; i < MB * KS * 64; i += kBlock) sA[i] = Af[i];
   const T* __restrict__ pA = (kALds ? (const T*)sA : Af) + lane;
 
-  const int xcd = blockIdx.x % 8, lb = blockIdx.x / 8, bpx = gridDim.x / 8;
-  for (int64_t it = lb; it < a.tiles_per_xcd; it += bpx) {
-    const int64_t tile = (int64_t)xcd * a.tiles_per_xcd + it;
-    if (tile >= a.tiles) break;
-    const int64_t e0 = tile * TE;
-    const int ne = (int)((K - e0) < TE ? (K - e0) : TE);
-    __syncthreads();
-    // phase 1: state tile, geometry, face table, normals; epilogue operands into registers
-    for (int idx = tid; idx < 4 * TE * NP; idx += kBlock) {
-      int qq = idx / (TE * NP), r = idx - qq * (TE * NP);
-      sU[qq][r] = r < ne * NP ? Vin[qq * KN + e0 * NP + r] : (T)0;
-    }
-    for (int idx = tid; idx < TE * 9; idx += kBlock) {
-      int e = idx / 9, c = idx - e * 9;
-      sG[e][c] = e < ne ? Jinv[(e0 + e) * 9 + c] : (T)0;
-    }
-    for (int idx = tid; idx < TE * NF; idx += kBlock) {
-      int e = idx / NF, j = idx - e * NF;
-      int2 nc = make_int2(-1, 0);
-      T n[3] = {0, 0, 0}, len = 1;
-      if (e < ne) {
-        nc = make_int2(a.nbr[(e0 + e) * NF + j], a.code[(e0 + e) * NF + j]);
-        const T* Ge = Jinv + (e0 + e) * 9;
-        for (int x = 0; x < 3; ++x) {
-          T sum = 0;
-          for (int r = 0; r < D; ++r) sum += Ge[r * 3 + x] * (T)a.refn[j][r];
-          n[x] = sum;
-        }
-        len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-      }
-      T il = (T)1 / len;
-      sN[idx][0] = n[0] * il;
-      sN[idx][1] = n[1] * il;
-      sN[idx][2] = n[2] * il;
-      sN[idx][3] = len * (T)a.fsc[j];
-      sNC[idx] = nc;
-    }
-    T pu[NE], pa[NE];
-    if (stage == 1 || stage == 2 || stage == 3) {
+  // Per-thread offsets that do not depend on the tile (every tile is full: the element count is
+  // padded to a multiple of TE and padding elements carry zero state and geometry).
+  int voff[NE];  // plane*PS + r for the thread's state vectors (add e0*NP)
+  int loff[NE];  // plane*TE*NP + r in LDS
 #pragma unroll
-      for (int t = 0; t < NE; ++t) {
-        const int idx = tid + t * kBlock;
-        pu[t] = 0;
-        pa[t] = 0;
-        if (idx < 4 * TE * NP) {
-          const int qq = idx / (TE * NP), r = idx - qq * (TE * NP);
-          if (r < ne * NP) {
-            const int64_t id = qq * KN + e0 * NP + r;
-            if (stage != 3) pu[t] = U[id];
-            pa[t] = Acc[id];
-          }
-        }
+  for (int t = 0; t < NE; ++t) {
+    const int v = tid + t * kBlock, qq = v / PV, r = (v - qq * PV) * VW;
+    voff[t] = qq * PS + r;
+    loff[t] = qq * TE * NP + r;
+  }
+
+  // Software pipeline: while tile t is processed, tile t+1's state, geometry and face geometry,
+  // and the (neighbour, code) pairs of the thread's face items are in flight into registers; the
+  // neighbour traces of tile t+1 are gathered right after tile t's flux phase.
+  VT pst[NE];
+  T pgeo[NGEO], pfg[NFC][4];
+  int2 inc[NI], cnc[NI];
+  T iqp[NI][4];
+  auto prefetch = [&](int e0) {
+#pragma unroll
+    for (int t = 0; t < NE; ++t)
+      if (tid + t * kBlock < 4 * PV) pst[t] = *(const VT*)(Vin + voff[t] + e0 * NP);
+#pragma unroll
+    for (int t = 0; t < NGEO; ++t) {
+      const int idx = tid + t * kBlock;
+      if (idx < TE * 9) pgeo[t] = Jinv[e0 * 9 + idx];
+    }
+#pragma unroll
+    for (int t = 0; t < NFC; ++t) {
+      const int idx = tid + t * kBlock;
+      if (idx < TE * NF)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) pfg[t][c] = Fg[(e0 * NF + idx) * 4 + c];
+    }
+#pragma unroll
+    for (int t = 0; t < NI; ++t) {
+      const int idx = tid + t * kBlock;
+      const int ej = e0 * NF + (idx / NFF) * NF + (idx % NFF) / NFP;
+      inc[t] = idx < TE * NFF ? make_int2(a.nbr[ej], a.code[ej]) : make_int2(-1, 0);
+    }
+  };
+  auto gather = [&]() {
+#pragma unroll
+    for (int t = 0; t < NI; ++t) {
+      const int k = ((tid + t * kBlock) % NFF) % NFP;
+      if (inc[t].x >= 0) {
+        const int pi = inc[t].x * NP + sVmap[(inc[t].y & 0xffff) * NFP + k];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) iqp[t][c] = Vin[c * PS + pi];
+      } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) iqp[t][c] = (T)0;
       }
     }
+  };
+
+  const int xcd = blockIdx.x % 8, lb = blockIdx.x / 8, bpx = gridDim.x / 8;
+  const int tpx = a.tiles_per_xcd, ntiles = (int)a.tiles;
+  auto valid = [&](int it) { return it < tpx && xcd * tpx + it < ntiles; };
+  __syncthreads();  // sVmap ready for the first gathers
+  if (valid(lb)) {
+    prefetch((xcd * tpx + lb) * TE);
+    gather();
+  }
+  for (int it = lb; valid(it); it += bpx) {
+    const int e0 = (xcd * tpx + it) * TE;
+    __syncthreads();  // the previous tile is done with LDS
+    // phase 1: commit the prefetched tile to LDS; epilogue operands and the next tile in flight
+#pragma unroll
+    for (int t = 0; t < NE; ++t)
+      if (tid + t * kBlock < 4 * PV) *(VT*)(&sU[0][0] + loff[t]) = pst[t];
+#pragma unroll
+    for (int t = 0; t < NGEO; ++t) {
+      const int idx = tid + t * kBlock;
+      if (idx < TE * 9) sG[idx / 9][idx % 9] = pgeo[t];
+    }
+#pragma unroll
+    for (int t = 0; t < NFC; ++t) {
+      const int idx = tid + t * kBlock;
+      if (idx < TE * NF)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sN[idx][c] = pfg[t][c];
+    }
+    VT pu[NE], pa[NE];
+    if (stage >= 1 && stage <= 3) {
+#pragma unroll
+      for (int t = 0; t < NE; ++t)
+        if (tid + t * kBlock < 4 * PV) {
+          const int id = voff[t] + e0 * NP;
+          if (stage != 3) pu[t] = *(const VT*)(U + id);
+          pa[t] = *(const VT*)(Acc + id);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NI; ++t) cnc[t] = inc[t];
+    const bool more = valid(it + bpx);
+    if (more) prefetch((xcd * tpx + it + bpx) * TE);
     __syncthreads();
     // phase 2: nodal normal numerical fluxes scaled by sJ/|detJ| -> sB[q][e*RB + jk]
 #pragma unroll
     for (int t = 0; t < NI; ++t) {
       const int idx = tid + t * kBlock;
       if (idx >= TE * NFF) break;
-      const int e = idx / NFF, jk = idx - e * NFF, j = jk / NFP, k = jk - j * NFP;
+      const int e = idx / NFF, jk = idx - e * NFF, j = jk / NFP;
       const int ej = e * NF + j;
-      T fl[4] = {0, 0, 0, 0};
-      const int2 nc = sNC[ej];
-      if (e < ne) {
-        const T n[3] = {sN[ej][0], sN[ej][1], sN[ej][2]};
-        const T fs = sN[ej][3];
-        const int on = sFmask[jk];
-        const T qm[4] = {sU[0][e * NP + on], sU[1][e * NP + on], sU[2][e * NP + on], sU[3][e * NP + on]};
-        const T vn = v0[0] * n[0] + v0[1] * n[1] + v0[2] * n[2];
-        const T unm = qm[1] * n[0] + qm[2] * n[1] + qm[3] * n[2];
-        if (nc.x >= 0) {
-          const int node = sVmap[(nc.y & 0xffff) * NFP + k];
-          const T hc = (T)0.5 * ((nc.y >> 16) & 1 ? -c0 : c0);
-          const int64_t pi = (int64_t)nc.x * NP + node;
-          const T qp[4] = {Vin[pi], Vin[KN + pi], Vin[2 * KN + pi], Vin[3 * KN + pi]};
-          const T unp = qp[1] * n[0] + qp[2] * n[1] + qp[3] * n[2];
-          fl[0] = (T)0.5 * ((vn * qm[0] + rc2 * unm) + (vn * qp[0] + rc2 * unp)) + hc * (qm[0] - qp[0]);
-          for (int c = 0; c < 3; ++c)
-            fl[1 + c] = (T)0.5 * ((vn * qm[1 + c] + qm[0] * irho * n[c]) + (vn * qp[1 + c] + qp[0] * irho * n[c])) +
-                        hc * (qm[1 + c] - qp[1 + c]);
-        } else if (nc.x == -2) {
-          const T gv[3] = {qm[1] - unm * n[0], qm[2] - unm * n[1], qm[3] - unm * n[2]};
-          const T ung = gv[0] * n[0] + gv[1] * n[1] + gv[2] * n[2];
-          fl[0] = vn * qm[0] + rc2 * ung;
-          for (int c = 0; c < 3; ++c) fl[1 + c] = vn * gv[c] + qm[0] * irho * n[c];
-        } else {
-          fl[0] = (T)0.25 * (c0 * qm[0] + rc2 * unm);
-          const T tt = (T)0.25 * (qm[0] * irho + c0 * unm);
-          for (int c = 0; c < 3; ++c) fl[1 + c] = tt * n[c];
-        }
-        for (int qq = 0; qq < 4; ++qq) fl[qq] *= fs;
+      const int2 nc = cnc[t];
+      const T n[3] = {sN[ej][0], sN[ej][1], sN[ej][2]};
+      const T fs = sN[ej][3];
+      const int on = e * NP + sFmask[jk];
+      const T qm[4] = {sU[0][on], sU[1][on], sU[2][on], sU[3][on]};
+      const T vn = v0[0] * n[0] + v0[1] * n[1] + v0[2] * n[2];
+      const T unm = qm[1] * n[0] + qm[2] * n[1] + qm[3] * n[2];
+      T fl[4];
+      if (nc.x >= 0) {
+        const T hc = (T)0.5 * ((nc.y >> 16) & 1 ? -c0 : c0);
+        const T qp[4] = {iqp[t][0], iqp[t][1], iqp[t][2], iqp[t][3]};
+        const T unp = qp[1] * n[0] + qp[2] * n[1] + qp[3] * n[2];
+        fl[0] = (T)0.5 * ((vn * qm[0] + rc2 * unm) + (vn * qp[0] + rc2 * unp)) + hc * (qm[0] - qp[0]);
+        for (int c = 0; c < 3; ++c)
+          fl[1 + c] = (T)0.5 * ((vn * qm[1 + c] + qm[0] * irho * n[c]) + (vn * qp[1 + c] + qp[0] * irho * n[c])) +
+                      hc * (qm[1 + c] - qp[1 + c]);
+      } else if (nc.x == -2) {
+        const T gv[3] = {qm[1] - unm * n[0], qm[2] - unm * n[1], qm[3] - unm * n[2]};
+        const T ung = gv[0] * n[0] + gv[1] * n[1] + gv[2] * n[2];
+        fl[0] = vn * qm[0] + rc2 * ung;
+        for (int c = 0; c < 3; ++c) fl[1 + c] = vn * gv[c] + qm[0] * irho * n[c];
+      } else {
+        fl[0] = (T)0.25 * (c0 * qm[0] + rc2 * unm);
+        const T tt = (T)0.25 * (qm[0] * irho + c0 * unm);
+        for (int c = 0; c < 3; ++c) fl[1 + c] = tt * n[c];
       }
-      for (int qq = 0; qq < 4; ++qq) sB[qq][e * RB + jk] = fl[qq];
+      for (int qq = 0; qq < 4; ++qq) sB[qq][e * RB + jk] = fs * fl[qq];
     }
+    if (more) gather();  // next tile's neighbour traces, in flight during the MFMA phase
     __syncthreads();
     // phase 3: per-lane coefficients Ft_r = alpha_r p + sum_x beta_rx v_x, then the MFMA chain
     T alpha[D], beta[D][3];
@@ -414,32 +459,33 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb) acc[mb] = Mfma<T>::mma(pA[(mb * KS + s) * 64], b, acc[mb]);
     }
-    __syncthreads();  // face fluxes consumed; sB now receives the results
+    __syncthreads();  // face fluxes consumed; sB now receives the results as [4][TE*NP]
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int i = 16 * mb + d_row<T>(lane, j);
-        if (i < NP) sB[q][el * RB + i] = acc[mb][j];
+        if (i < NP) sR[q * TE * NP + el * NP + i] = acc[mb][j];
       }
     __syncthreads();
-    // phase 4: coalesced time-integration epilogue
+    // phase 4: time-integration epilogue on 16-byte vectors
+    const T s6 = (T)(1.0 / 6.0), s3 = (T)(1.0 / 3.0), s2 = (T)0.5;
 #pragma unroll
     for (int t = 0; t < NE; ++t) {
-      const int idx = tid + t * kBlock;
-      if (idx >= 4 * TE * NP) break;
-      const int qq = idx / (TE * NP), r = idx - qq * (TE * NP), e = r / NP, i = r - e * NP;
-      if (e >= ne) continue;
-      const T kq = dt * sB[qq][e * RB + i];
-      const int64_t id = qq * KN + e0 * NP + r;
-      const T s6 = (T)(1.0 / 6.0), s3 = (T)(1.0 / 3.0), s2 = (T)0.5;
+      if (tid + t * kBlock >= 4 * PV) break;
+      const VT kq = dt * *(const VT*)(sR + loff[t]);
+      const int id = voff[t] + e0 * NP;
       switch (stage) {
-        case 0: { const T u = sU[qq][r]; Acc[id] = u + kq * s6; Vout[id] = u + kq * s2; } break;
-        case 1: Acc[id] = pa[t] + kq * s3; Vout[id] = pu[t] + kq * s2; break;
-        case 2: Acc[id] = pa[t] + kq * s3; Vout[id] = pu[t] + kq; break;
-        case 3: U[id] = pa[t] + kq * s6; break;
-        case 4: Vout[id] = kq; break;
-        default: Vout[id] = sU[qq][r] + kq; break;
+        case 0: {
+          const VT u = *(const VT*)(&sU[0][0] + loff[t]);
+          *(VT*)(Acc + id) = u + kq * s6;
+          *(VT*)(Vout + id) = u + kq * s2;
+        } break;
+        case 1: *(VT*)(Acc + id) = pa[t] + kq * s3; *(VT*)(Vout + id) = pu[t] + kq * s2; break;
+        case 2: *(VT*)(Acc + id) = pa[t] + kq * s3; *(VT*)(Vout + id) = pu[t] + kq; break;
+        case 3: *(VT*)(U + id) = pa[t] + kq * s6; break;
+        case 4: *(VT*)(Vout + id) = kq; break;
+        default: *(VT*)(Vout + id) = *(const VT*)(&sU[0][0] + loff[t]) + kq; break;
       }
     }
   }
@@ -504,7 +550,7 @@ void launch_mfma(StageArgs& a, hipStream_t s) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stage_mfma<T, D, P>, kBlock, 0) != hipSuccess || occ < 1)
       occ = 1;
   }
-  a.tiles = (a.K + TE - 1) / TE;
+  a.tiles = a.Kpad / TE;
   a.tiles_per_xcd = (int)((a.tiles + 7) / 8);
   int64_t per_xcd_cap = (int64_t)(num_cus() / 8) * occ;
   int bpx = (int)std::min<int64_t>(a.tiles_per_xcd, per_xcd_cap);
@@ -549,7 +595,7 @@ void launch_source(Ctx& c, const Ctx::Src& src, double value, hipStream_t st) {
 
 bool check_finite(Ctx& c) {
   DGK_HIP(hipMemsetAsync(c.flag, 0, sizeof(int), c.stream));
-  int64_t n = 4 * c.K * c.Np;
+  int64_t n = 4 * c.PS;
   if (c.precision == DGK_FP64)
     hipLaunchKernelGGL(k_finite<double>, dim3(1024), dim3(256), 0, c.stream, (const double*)c.U, n, c.flag);
   else

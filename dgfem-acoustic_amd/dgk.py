@@ -14,7 +14,7 @@ import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 HOST_LIB = os.path.join(PKG_DIR, "libdgkhost.so")
-DEV_LIB = os.path.join(PKG_DIR, "libdgk.so")
+DEV_LIB = os.environ.get("DGK_LIBDGK") or os.path.join(PKG_DIR, "libdgk.so")
 
 _host = None
 _dev = None
