@@ -55,8 +55,9 @@ def algorithmic(m, info, word):
     return bytes_per, flops_per
 
 
-def run_form(m, ph, dt, u0, precision, steps, warmup):
-    s = dgk.Solver(m, ph["c0"], ph["rho0"], v0=ph["v0"], precision=precision, flux_sign=dgk.SIGN_STABLE)
+def run_form(m, ph, dt, u0, precision, steps, warmup, variant=0, fp32_flags=0):
+    s = dgk.Solver(m, ph["c0"], ph["rho0"], v0=ph["v0"], precision=precision, flux_sign=dgk.SIGN_STABLE,
+                   variant=variant, fp32_flags=fp32_flags)
     s.set_state(u0)
     s.rk4(warmup, dt)
     s.sync()
@@ -93,6 +94,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,fp32", help="comma list of fp64,fp32")
+    ap.add_argument("--variant", type=int, default=0, help="stage-kernel variant (0 = default)")
+    ap.add_argument("--fp32-flags", type=int, default=0, help="FP32 form switches (dgk.FP32_* bits, 0 = full form)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="first run a plain device copy of MB MiB (reference dispatch for HBM byte counters)")
     args = ap.parse_args()
@@ -118,7 +121,8 @@ def main():
     for name, prec in forms:
         print(f"[bench] {name}: {args.warmup} warmup + {args.steps} timed RK4 steps, K={m.K}", file=sys.stderr,
               flush=True)
-        s, wall, kern_ms, state, finite, nsteps_total = run_form(m, ph, dt, u0, prec, args.steps, args.warmup)
+        s, wall, kern_ms, state, finite, nsteps_total = run_form(m, ph, dt, u0, prec, args.steps, args.warmup,
+                                                                 args.variant, args.fp32_flags)
         info = s.info
         rate = dof * 4 * args.steps / wall
         word = 8 if prec == dgk.FP64 else 4
@@ -143,7 +147,8 @@ def main():
     def roof(r):
         return {"bound": "hbm", "achieved": round(r["ach_bw"] / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": round(r["ach_bw"] / HBM_PEAK, 4), "traffic": None,
-                "kernel": f"k_stage_lds ({r['variant']})", "kernel_avg_us": round(r["kern_ms"] * 1e3, 2),
+                "kernel": ("k_stage_lds" if r["variant"] == "lds-tile" else "k_stage_mfma") + f" ({r['variant']})",
+                "kernel_avg_us": round(r["kern_ms"] * 1e3, 2),
                 "algorithmic_bytes_per_dof_stage": round(r["bpd"], 2),
                 "flop_per_dof_stage": round(r["fpd"], 1)}
 
@@ -163,7 +168,8 @@ def main():
         "config": {"workload": "cube3d-p3: unit cube, 32^3 Kuhn cells (196,608 tets), 10% jitter, order 3, "
                                "4 fields, v0=(50,0,0), absorbing walls, Gaussian pulse",
                    "elements": m.K, "order": 3, "dofs": dof, "dt": dt, "flux_sign": "stable",
-                   "form": "fp32 state and arithmetic, fp64 geometry rounded once"},
+                   "form": "fp32 MFMA arithmetic, fp32 solution with bf16 low word (compensated), fp64 geometry "
+                           "rounded once", "fp32_flags": args.fp32_flags},
         "fp32_rel_l2_vs_fp64": {"after_steps": r32["steps_total"], **errs},
         "fp64_plain": {"value": r64["rate"], "ms_per_step": r64["wall"] * 1e3 / args.steps, "dtype": "fp64",
                        "roofline": roof(r64), "finite": r64["finite"]},

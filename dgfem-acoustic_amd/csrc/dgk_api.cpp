@@ -1,5 +1,6 @@
 // C ABI of libdgk (include/dgk.h): context setup, state transfer and time stepping.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <unordered_map>
@@ -17,15 +18,17 @@ Ctx::~Ctx() {
   if (stream) (void)hipStreamDestroy(stream);
 }
 
-namespace {
-
 thread_local std::string g_err;
+
+namespace {
 
 dgk_status fail(Ctx* c, dgk_status st, const std::string& msg) {
   g_err = msg;
   if (c) c->err = msg;
   return st;
 }
+
+}  // namespace
 
 template <class F>
 dgk_status guarded(Ctx* c, F&& f) {
@@ -46,6 +49,8 @@ dgk_status guarded(Ctx* c, F&& f) {
     return fail(c, DGK_ERR_INVALID, e.what());
   }
 }
+
+namespace {
 
 // derivative of the equispaced order-p vertex Lagrange function psi(l) = prod_k (p l - k)/(k+1)
 double dpsi(int p, double l) {
@@ -114,6 +119,36 @@ void upload_ops(Ctx& c, const RefElem& R, const std::vector<int>& vmap) {
       }
   c.ops.Amf = c.alloc(A.size() * sizeof(T));
   DGK_HIP(hipMemcpy(c.ops.Amf, A.data(), A.size() * sizeof(T), hipMemcpyHostToDevice));
+  if (sizeof(T) == 4 && c.oplo) {
+    // bf16 residuals: lane l of correction MFMA (mb, t) holds slots i = 0..7 for k-steps 8t+i
+    const int NLO = (KS + 7) / 8;
+    std::vector<uint16_t> lo((size_t)MB * NLO * 64 * 8, 0);
+    auto to_bf16 = [](float f) {
+      uint32_t u;
+      memcpy(&u, &f, 4);
+      u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+      return (uint16_t)(u >> 16);
+    };
+    for (int mb = 0; mb < MB; ++mb)
+      for (int s = 0; s < KS; ++s)
+        for (int l = 0; l < 64; ++l) {
+          int i = 16 * mb + (l % 16), g = l / 16;
+          double v = 0;
+          if (i < Np) {
+            if (s < d * NG) {
+              int r = s % d, n = 4 * (s / d) + g;
+              if (n < Np) v = R.W[(r * Np + i) * Np + n];
+            } else {
+              int jk = 4 * (s - d * NG) + g;
+              if (jk < NFF) v = -R.LIFT[i * NFF + jk];
+            }
+          }
+          float hi = (float)v;
+          lo[(((size_t)mb * NLO + s / 8) * 64 + l) * 8 + s % 8] = to_bf16((float)(v - (double)hi));
+        }
+    c.ops.Alo = c.alloc(lo.size() * sizeof(uint16_t));
+    DGK_HIP(hipMemcpy(c.ops.Alo, lo.data(), lo.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  }
 }
 
 // vmap[(jn*nperm + perm)*Nfp + k]: neighbour element node matching own face node k when the
@@ -145,6 +180,8 @@ std::vector<int> make_vmap(const RefElem& R, int nperm) {
   return vm;
 }
 
+}  // namespace
+
 void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* o) {
   if (!m || !ph) throw InvalidArg("mesh and phys must not be NULL");
   if (m->dim < 1 || m->dim > 3) throw Unsupported("element dimension must be 1, 2 or 3");
@@ -162,6 +199,12 @@ void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* 
   c.variant = o ? o->variant : 0;
   if (c.variant == 0) c.variant = kVariantMfma;
   if (c.variant != kVariantLds && c.variant != kVariantMfma) throw InvalidArg("unknown stage-kernel variant");
+  const int flags = o ? o->fp32_flags : 0;
+  if (flags & ~(DGK_FP32_PLAIN_STATE | DGK_FP32_OPERATOR_CORRECTION)) throw InvalidArg("unknown fp32_flags bits");
+  // the thread-per-node kernel implements the plain FP32 form only
+  const bool f32mfma = (o ? o->precision : DGK_FP64) == DGK_FP32 && c.variant == kVariantMfma;
+  c.comp = f32mfma && !(flags & DGK_FP32_PLAIN_STATE);
+  c.oplo = f32mfma && (flags & DGK_FP32_OPERATOR_CORRECTION);
   if (c.precision != DGK_FP64 && c.precision != DGK_FP32) throw InvalidArg("precision must be DGK_FP64 or DGK_FP32");
   if (c.sign_mode != DGK_SIGN_REFERENCE && c.sign_mode != DGK_SIGN_STABLE) throw InvalidArg("bad flux_sign");
   c.tsize = c.precision == DGK_FP64 ? 8 : 4;
@@ -240,6 +283,10 @@ void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* 
   c.perm = (int32_t*)c.alloc(KP * c.Nf * sizeof(int32_t));
   c.flag = (int*)c.alloc(sizeof(int));
   for (void* p : {c.U, c.A, c.V0, c.V1}) DGK_HIP(hipMemsetAsync(p, 0, n * c.tsize, c.stream));
+  if (c.comp) {
+    c.Ulo = (uint16_t*)c.alloc(n * sizeof(uint16_t));
+    DGK_HIP(hipMemsetAsync(c.Ulo, 0, n * sizeof(uint16_t), c.stream));
+  }
   DGK_HIP(hipMemsetAsync(c.Jinv, 0, KP * 9 * c.tsize, c.stream));
   DGK_HIP(hipMemsetAsync(c.fgeom, 0, KP * c.Nf * 4 * c.tsize, c.stream));
   DGK_HIP(hipMemsetAsync(c.nbr, 0xff, KP * c.Nf * sizeof(int32_t), c.stream));
@@ -276,6 +323,8 @@ void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* 
   c.F = (K * c.Nf + nbnd) / 2;
 }
 
+namespace {
+
 StageArgs base_args(Ctx& c) {
   StageArgs a{};
   a.K = c.K;
@@ -283,6 +332,7 @@ StageArgs base_args(Ctx& c) {
   a.PS = c.PS;
   a.U = c.U;
   a.A = c.A;
+  a.Ulo = c.comp ? c.Ulo : nullptr;
   a.Jinv = c.Jinv;
   a.fgeom = c.fgeom;
   a.nbr = c.nbr;
@@ -292,6 +342,7 @@ StageArgs base_args(Ctx& c) {
   a.fmask = c.ops.fmask;
   a.vmap = c.ops.vmap;
   a.Amf = c.ops.Amf;
+  a.Alo = c.ops.Alo;
   for (int j = 0; j < kMaxNF; ++j) {
     for (int x = 0; x < 3; ++x) a.refn[j][x] = c.refn[j][x];
     a.fsc[j] = c.fsc[j];
@@ -302,8 +353,10 @@ StageArgs base_args(Ctx& c) {
   return a;
 }
 
+}  // namespace
+
 // Launch the four RK4 stages of one step on stream s.
-void rk4_step(Ctx& c, double dt, hipStream_t s, hipEvent_t* ev = nullptr) {
+void rk4_step(Ctx& c, double dt, hipStream_t s, hipEvent_t* ev) {
   StageArgs a = base_args(c);
   a.dt = dt;
   void* in[4] = {c.U, c.V0, c.V1, c.V0};
@@ -317,6 +370,8 @@ void rk4_step(Ctx& c, double dt, hipStream_t s, hipEvent_t* ev = nullptr) {
     if (ev) DGK_HIP(hipEventRecord(ev[2 * st + 1], s));
   }
 }
+
+namespace {
 
 void apply_sources(Ctx& c, double t) {
   for (auto& src : c.sources)
@@ -384,6 +439,8 @@ void euler_impl(Ctx& c, int nsteps, double dt, const double* ts) {
   }
 }
 
+}  // namespace
+
 double* staging(Ctx& c) {
   if (!c.stage_d) {
     c.stage_d = (double*)c.alloc(4 * c.PS * sizeof(double));
@@ -391,6 +448,8 @@ double* staging(Ctx& c) {
   }
   return c.stage_d;
 }
+
+namespace {
 
 // Host u[4][K*Np] (FP64) <-> device planes with stride PS in the state precision.
 void upload_planes(Ctx& c, const double* u, void* dst) {
@@ -401,7 +460,7 @@ void upload_planes(Ctx& c, const double* u, void* dst) {
   }
   double* s = staging(c);
   DGK_HIP(hipMemcpy2DAsync(s, c.PS * sizeof(double), u, row, row, 4, hipMemcpyHostToDevice, c.stream));
-  convert_state(c, s, dst, 4 * c.PS);
+  convert_state(c, s, dst, 4 * c.PS, dst == c.U && c.comp ? c.Ulo : nullptr);
 }
 
 void download_planes(Ctx& c, const void* src, double* u) {
@@ -411,7 +470,7 @@ void download_planes(Ctx& c, const void* src, double* u) {
     return;
   }
   double* s = staging(c);
-  convert_state_back(c, src, s, 4 * c.PS);
+  convert_state_back(c, src, s, 4 * c.PS, src == c.U && c.comp ? c.Ulo : nullptr);
   DGK_HIP(hipMemcpy2DAsync(u, row, s, c.PS * sizeof(double), row, 4, hipMemcpyDeviceToHost, c.stream));
 }
 
@@ -581,5 +640,276 @@ dgk_status dgk_rk4_timed(dgk_ctx* c, int nsteps, double dt, double* stage_ms, do
 }
 
 const char* dgk_variant_name(int v) { return variant_name(v); }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------- batches
+struct dgk_batch {
+  struct Group {
+    dgk::Ctx* ctx = nullptr;
+    std::vector<int> sims;
+    std::vector<int64_t> off;  // element offset of each member in the union
+  };
+  std::vector<Group> groups;
+  std::vector<std::pair<int, int>> where;  // sim -> (group, member)
+  std::vector<int64_t> K;                  // elements per sim
+  hipStream_t main = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> joins;
+  hipGraphExec_t graph = nullptr;
+  int graph_steps = 0, graph_schedule = -1;
+  std::string err;
+  ~dgk_batch() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    for (auto& g : groups) delete g.ctx;
+    for (auto e : joins) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    if (main) (void)hipStreamDestroy(main);
+  }
+};
+
+namespace {
+
+dgk_status bfail(dgk_batch* b, dgk_status st, const std::string& msg) {
+  dgk::g_err = msg;
+  if (b) b->err = msg;
+  return st;
+}
+
+template <class F>
+dgk_status bguard(dgk_batch* b, F&& f) {
+  dgk::Ctx tmp;
+  dgk_status st = dgk::guarded(&tmp, f);
+  if (st != DGK_OK) return bfail(b, st, tmp.err);
+  return DGK_OK;
+}
+
+void batch_create_impl(dgk_batch& B, int n, const dgk_mesh* meshes, const dgk_phys* phys, const double* dt,
+                       const dgk_opts* opts) {
+  using namespace dgk;
+  if (n <= 0 || !meshes || !phys || !dt) throw InvalidArg("empty batch");
+  B.where.resize(n);
+  B.K.resize(n);
+  // Group key: dim, order, physics. A group closes when adding a member would break the per-context
+  // limits (32-bit plane indexing, 2^21 vertices in 3D); the next member of that key opens a new one.
+  std::vector<std::vector<int>> members;
+  std::vector<std::array<double, 7>> keys;
+  std::vector<int64_t> gK, gV;
+  for (int i = 0; i < n; ++i) {
+    const dgk_mesh& m = meshes[i];
+    if (!(dt[i] > 0)) throw InvalidArg("batch time steps must be positive");
+    if (m.dim < 1 || m.dim > 3) throw Unsupported("element dimension must be 1, 2 or 3");
+    if (m.order < 1 || m.order > 5) throw Unsupported("element order must be 1..5");
+    if (m.num_elements <= 0 || !m.node_xyz || !m.node_tags) throw InvalidArg("empty mesh in batch");
+    const int Np = num_nodes(m.dim, m.order);
+    std::vector<int64_t> corners;
+    corners.reserve(m.num_elements * (m.dim + 1));
+    for (int64_t e = 0; e < m.num_elements; ++e)
+      for (int v = 0; v <= m.dim; ++v) corners.push_back(m.node_tags[e * Np + v]);
+    std::sort(corners.begin(), corners.end());
+    const int64_t nv = std::unique(corners.begin(), corners.end()) - corners.begin();
+    std::array<double, 7> k{(double)m.dim, (double)m.order, phys[i].c0, phys[i].rho0,
+                            phys[i].v0[0], phys[i].v0[1], phys[i].v0[2]};
+    size_t gi = 0;
+    while (gi < keys.size() &&
+           (keys[gi] != k || 4 * (gK[gi] + m.num_elements + kTilePad) * Np >= ((int64_t)1 << 31) ||
+            (m.dim == 3 && gV[gi] + nv >= ((int64_t)1 << 21))))
+      ++gi;
+    if (gi == keys.size()) {
+      keys.push_back(k);
+      members.emplace_back();
+      gK.push_back(0);
+      gV.push_back(0);
+    }
+    members[gi].push_back(i);
+    gK[gi] += m.num_elements;
+    gV[gi] += nv;
+  }
+  for (size_t gi = 0; gi < keys.size(); ++gi) {
+    dgk_batch::Group G;
+    G.sims = members[gi];
+    const dgk_mesh& m0 = meshes[G.sims[0]];
+    const int Np = num_nodes(m0.dim, m0.order);
+    int64_t Ktot = 0;
+    for (int i : G.sims) Ktot += meshes[i].num_elements;
+    std::vector<double> xyz;
+    std::vector<int64_t> tags;
+    std::vector<uint8_t> bc;
+    xyz.reserve(Ktot * Np * 3);
+    tags.reserve(Ktot * Np);
+    bc.reserve(Ktot * Np);
+    int64_t tag_base = 0;
+    for (size_t k = 0; k < G.sims.size(); ++k) {
+      const dgk_mesh& m = meshes[G.sims[k]];
+      G.off.push_back((int64_t)tags.size() / Np);
+      int64_t mx = 0;
+      for (int64_t j = 0; j < m.num_elements * Np; ++j) {
+        tags.push_back(m.node_tags[j] + tag_base);  // tags stay unique across members
+        mx = std::max(mx, m.node_tags[j]);
+        bc.push_back(m.node_bc ? m.node_bc[j] : 0);
+      }
+      xyz.insert(xyz.end(), m.node_xyz, m.node_xyz + m.num_elements * Np * 3);
+      tag_base += mx + 1;
+      B.where[G.sims[k]] = {(int)B.groups.size(), (int)k};
+      B.K[G.sims[k]] = m.num_elements;
+    }
+    dgk_mesh um{m0.dim, m0.order, Ktot, xyz.data(), tags.data(), bc.data()};
+    G.ctx = new Ctx();
+    B.groups.push_back(G);  // owned by B from here on
+    create_impl(*B.groups.back().ctx, &um, &phys[G.sims[0]], opts);
+    Ctx& c = *B.groups.back().ctx;
+    for (size_t k = 0; k < G.sims.size(); ++k) scale_geometry(c, G.off[k], B.K[G.sims[k]], dt[G.sims[k]]);
+    DGK_HIP(hipStreamSynchronize(c.stream));
+  }
+  DGK_HIP(hipStreamCreateWithFlags(&B.main, hipStreamNonBlocking));
+  DGK_HIP(hipEventCreateWithFlags(&B.fork, hipEventDisableTiming));
+  B.joins.resize(B.groups.size());
+  for (auto& e : B.joins) DGK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+}
+
+// stage launches of nsteps RK4 steps for every group, on the groups' streams (parallel) or main
+void batch_enqueue(dgk_batch& B, int nsteps, int schedule) {
+  using namespace dgk;
+  if (schedule == 0) {
+    DGK_HIP(hipEventRecord(B.fork, B.main));
+    for (size_t g = 0; g < B.groups.size(); ++g) {
+      Ctx& c = *B.groups[g].ctx;
+      DGK_HIP(hipStreamWaitEvent(c.stream, B.fork, 0));
+      for (int i = 0; i < nsteps; ++i) rk4_step(c, 1.0, c.stream);
+      DGK_HIP(hipEventRecord(B.joins[g], c.stream));
+      DGK_HIP(hipStreamWaitEvent(B.main, B.joins[g], 0));
+    }
+  } else {
+    for (int i = 0; i < nsteps; ++i)
+      for (auto& G : B.groups) rk4_step(*G.ctx, 1.0, B.main);
+  }
+}
+
+void batch_rk4_impl(dgk_batch& B, int nsteps, int schedule) {
+  using namespace dgk;
+  if (nsteps < 0 || schedule < 0 || schedule > 1) throw InvalidArg("bad batch step arguments");
+  for (auto& G : B.groups)  // the batch stream runs after anything queued on the contexts
+    DGK_HIP(hipStreamSynchronize(G.ctx->stream));
+  // Up to kMaxGraphSteps steps of every group are captured as one graph and replayed; a
+  // capture is reused while the step count and schedule stay the same.
+  constexpr int kMaxGraphSteps = 1000;
+  const int chunk = std::min(nsteps, kMaxGraphSteps);
+  int done = 0;
+  while (chunk > 0 && nsteps - done >= chunk) {
+    if (!B.graph || B.graph_steps != chunk || B.graph_schedule != schedule) {
+      if (B.graph) { DGK_HIP(hipGraphExecDestroy(B.graph)); B.graph = nullptr; }
+      hipGraph_t gr;
+      DGK_HIP(hipStreamBeginCapture(B.main, hipStreamCaptureModeGlobal));
+      try {
+        batch_enqueue(B, chunk, schedule);
+      } catch (...) {
+        hipGraph_t junk;
+        (void)hipStreamEndCapture(B.main, &junk);
+        throw;
+      }
+      DGK_HIP(hipStreamEndCapture(B.main, &gr));
+      hipError_t e = hipGraphInstantiate(&B.graph, gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      DGK_HIP(e);
+      B.graph_steps = chunk;
+      B.graph_schedule = schedule;
+    }
+    DGK_HIP(hipGraphLaunch(B.graph, B.main));
+    done += chunk;
+  }
+  if (done < nsteps) batch_enqueue(B, nsteps - done, schedule);
+}
+
+}  // namespace
+
+extern "C" {
+
+dgk_status dgk_batch_create(int n, const dgk_mesh* meshes, const dgk_phys* phys, const double* dt,
+                            const dgk_opts* opts, dgk_batch** out) {
+  if (!out) return bfail(nullptr, DGK_ERR_INVALID, "out must not be NULL");
+  *out = nullptr;
+  dgk_batch* b = new dgk_batch();
+  dgk_status st = bguard(b, [&] { batch_create_impl(*b, n, meshes, phys, dt, opts); });
+  if (st != DGK_OK) {
+    delete b;
+    return st;
+  }
+  *out = b;
+  return DGK_OK;
+}
+
+dgk_status dgk_batch_destroy(dgk_batch* b) {
+  delete b;
+  return DGK_OK;
+}
+
+const char* dgk_batch_last_error(const dgk_batch* b) { return b ? b->err.c_str() : dgk::g_err.c_str(); }
+
+}  // extern "C"
+
+namespace {
+
+// Copy simulation `sim`'s state between host u[4][K_i*Np] (FP64) and its slice of the group planes.
+dgk_status batch_state_io(dgk_batch* b, int sim, double* u, bool to_device) {
+  if (!b || !u || sim < 0 || sim >= (int)b->where.size()) return bfail(b, DGK_ERR_INVALID, "bad batch state arguments");
+  return bguard(b, [&] {
+    auto w = b->where[sim];
+    dgk::Ctx& c = *b->groups[w.first].ctx;
+    DGK_HIP(hipStreamSynchronize(b->main));
+    const int64_t off = b->groups[w.first].off[w.second] * c.Np, n = b->K[sim] * c.Np;
+    const bool f64 = c.precision == DGK_FP64;
+    double* s = f64 ? nullptr : dgk::staging(c);
+    for (int q = 0; q < 4; ++q) {
+      const int64_t o = q * c.PS + off;
+      char* planes = (char*)c.U + o * c.tsize;
+      double* host = u + q * n;
+      if (to_device) {
+        DGK_HIP(hipMemcpyAsync(f64 ? (void*)planes : (void*)(s + o), host, n * sizeof(double), hipMemcpyHostToDevice,
+                               c.stream));
+        if (!f64) dgk::convert_state(c, s + o, planes, n, c.comp ? c.Ulo + o : nullptr);
+      } else {
+        if (!f64) dgk::convert_state_back(c, planes, s + o, n, c.comp ? c.Ulo + o : nullptr);
+        DGK_HIP(hipMemcpyAsync(host, f64 ? (const void*)planes : (const void*)(s + o), n * sizeof(double),
+                               hipMemcpyDeviceToHost, c.stream));
+      }
+    }
+    DGK_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+}  // namespace
+
+extern "C" {
+
+dgk_status dgk_batch_set_state(dgk_batch* b, int sim, const double* u) {
+  return batch_state_io(b, sim, const_cast<double*>(u), true);
+}
+
+dgk_status dgk_batch_get_state(dgk_batch* b, int sim, double* u) { return batch_state_io(b, sim, u, false); }
+
+dgk_status dgk_batch_rk4(dgk_batch* b, int nsteps, int schedule) {
+  if (!b) return bfail(nullptr, DGK_ERR_INVALID, "NULL batch");
+  return bguard(b, [&] { batch_rk4_impl(*b, nsteps, schedule); });
+}
+
+dgk_status dgk_batch_sync(dgk_batch* b) {
+  if (!b) return bfail(nullptr, DGK_ERR_INVALID, "NULL batch");
+  return bguard(b, [&] {
+    DGK_HIP(hipStreamSynchronize(b->main));
+    for (auto& G : b->groups) DGK_HIP(hipStreamSynchronize(G.ctx->stream));
+  });
+}
+
+dgk_status dgk_batch_groups(const dgk_batch* b, int* groups, int* dim, int* order, int64_t* elements, int* sims) {
+  if (!b) return bfail(nullptr, DGK_ERR_INVALID, "NULL batch");
+  if (groups) *groups = (int)b->groups.size();
+  for (size_t g = 0; g < b->groups.size(); ++g) {
+    if (dim) dim[g] = b->groups[g].ctx->dim;
+    if (order) order[g] = b->groups[g].ctx->p;
+    if (elements) elements[g] = b->groups[g].ctx->K;
+    if (sims) sims[g] = (int)b->groups[g].sims.size();
+  }
+  return DGK_OK;
+}
 
 }  // extern "C"

@@ -46,6 +46,7 @@ struct DevOps {
   int* fmask = nullptr;  // [Nf][Nfp]
   int* vmap = nullptr;   // [Nf*nperm][Nfp] neighbour element node of own face node k
   void* Amf = nullptr;   // [MB][KS][64] MFMA A fragments of [W_1..W_d | -LIFT] in k-step order
+  void* Alo = nullptr;   // FP32 only: [MB][ceil(KS/8)][64][8] bf16 residuals A - fl32(A)
 };
 
 // Per-stage kernel arguments (precision-agnostic pointers).
@@ -57,6 +58,7 @@ struct StageArgs {
   void* Vout;
   void* U;
   void* A;
+  void* Ulo;              // FP32 compensated form: bf16 low word of U (uint16 bits), else null
   const void* Jinv;       // [K][9]
   const void* fgeom;      // [K][Nf][4]: unit outward normal, sJ/|detJ|
   const int32_t* nbr;     // [K*Nf]
@@ -66,6 +68,7 @@ struct StageArgs {
   const int* fmask;
   const int* vmap;
   const void* Amf;
+  const void* Alo;
   double refn[kMaxNF][3];  // outward reference-face normals
   double fsc[kMaxNF];      // reference face measure / face reference-element measure
   double dt, c0, rho0, v0[3];
@@ -85,6 +88,9 @@ struct Ctx {
   size_t tsize = 8;  // bytes per state value
   // device buffers
   void *U = nullptr, *A = nullptr, *V0 = nullptr, *V1 = nullptr, *Jinv = nullptr, *fgeom = nullptr;
+  uint16_t* Ulo = nullptr;  // FP32 compensated form: bf16 low word of U
+  bool comp = false;        // FP32: compensated solution (U, Ulo)
+  bool oplo = false;        // FP32: bf16 operator-residual correction on MFMA
   int32_t *nbr = nullptr, *code = nullptr, *nface = nullptr, *perm = nullptr;
   double* stage_d = nullptr;  // FP64 staging for state transfer in FP32 mode
   int* flag = nullptr;
@@ -121,14 +127,23 @@ struct Ctx {
 void build_connectivity(Ctx& c, const double* vxyz_dev /* K*(dim+1)*3 */, const int32_t* vid_host /* K*(dim+1) */,
                         const uint8_t* bc_first_host /* K*Nf: code of each face's first node */, double sigma);
 void build_geometry(Ctx& c, const double* vxyz_dev);
+void scale_geometry(Ctx& c, int64_t e0, int64_t n, double s);
 // stage.hip
 void launch_stage(Ctx& c, StageArgs& a, hipStream_t s);
 void launch_source(Ctx& c, const Ctx::Src& s, double value, hipStream_t st);
 bool check_finite(Ctx& c);
-void convert_state(Ctx& c, const double* src_dev, void* dst_dev, int64_t n);       // FP64 -> state type
-void convert_state_back(Ctx& c, const void* src_dev, double* dst_dev, int64_t n);  // state type -> FP64
+// FP64 -> state type; in the FP32 compensated form lo (if given) receives the bf16 low words
+void convert_state(Ctx& c, const double* src_dev, void* dst_dev, int64_t n, uint16_t* lo = nullptr);
+// state type -> FP64, adding the low words when lo is given
+void convert_state_back(Ctx& c, const void* src_dev, double* dst_dev, int64_t n, const uint16_t* lo = nullptr);
 const char* variant_name(int v);
 void mfma_shape(int dim, int p, int* KS, int* MB);
+// dgk_api.cpp
+extern thread_local std::string g_err;
+void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* o);
+void rk4_step(Ctx& c, double dt, hipStream_t s, hipEvent_t* ev = nullptr);
+double* staging(Ctx& c);
+
 
 }  // namespace dgk
 

@@ -275,4 +275,30 @@ void build_geometry(Ctx& c, const double* vxyz_dev) {
   if (herr) throw MeshError("degenerate element (zero Jacobian)");
 }
 
+namespace {
+template <typename T>
+__global__ void k_scale_geometry(int64_t e0, int64_t n, int NF, double s, T* __restrict__ Jinv,
+                                 T* __restrict__ fgeom) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t e = e0 + i;
+  for (int c = 0; c < 9; ++c) Jinv[e * 9 + c] = (T)((double)Jinv[e * 9 + c] * s);
+  for (int j = 0; j < NF; ++j) fgeom[(e * NF + j) * 4 + 3] = (T)((double)fgeom[(e * NF + j) * 4 + 3] * s);
+}
+}  // namespace
+
+// Multiply the volume (J^-1) and face (sJ/|detJ|) scales of elements [e0, e0+n) by s: with s = dt
+// the stage operator returns dt*L(u) for a launch time step of 1.
+void scale_geometry(Ctx& c, int64_t e0, int64_t n, double s) {
+  if (n <= 0) return;
+  unsigned nb = (unsigned)((n + 255) / 256);
+  if (c.precision == DGK_FP64)
+    hipLaunchKernelGGL(k_scale_geometry<double>, dim3(nb), dim3(256), 0, c.stream, e0, n, c.Nf, s, (double*)c.Jinv,
+                       (double*)c.fgeom);
+  else
+    hipLaunchKernelGGL(k_scale_geometry<float>, dim3(nb), dim3(256), 0, c.stream, e0, n, c.Nf, s, (float*)c.Jinv,
+                       (float*)c.fgeom);
+  DGK_HIP(hipGetLastError());
+}
+
 }  // namespace dgk

@@ -21,6 +21,7 @@
 // and walks tiles in an XCD-aware order (each XCD sweeps one contiguous slab of the mesh,
 // so neighbour traces are reused from that XCD's L2).
 #include <algorithm>
+#include <type_traits>
 
 #include "dgk_internal.hpp"
 
@@ -214,6 +215,17 @@ struct Mfma<float> {
   }
 };
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+// bf16 16x16x32 MFMA into an FP32 accumulator; a no-op for other accumulator types (never called)
+template <typename VV>
+__device__ __forceinline__ VV mma_bf16(bf16x8, bf16x8, VV c) {
+  return c;
+}
+template <>
+__device__ __forceinline__ Mfma<float>::V mma_bf16(bf16x8 a, bf16x8 b, Mfma<float>::V c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
 template <int D, int P>
 struct MfmaShape {
   using S = Sz<D, P>;
@@ -224,6 +236,7 @@ struct MfmaShape {
   static constexpr int MB = (NP + 15) / 16;   // 16-row blocks
   static constexpr int TE = 16;               // elements per workgroup tile
   static constexpr int RB = NFF > NP ? NFF : NP;
+  static constexpr int NLO = (KS + 7) / 8;    // bf16 low-order correction MFMAs per m-block
 };
 
 // Output mapping of the 16x16x4 accumulator: row of register j in lane l (column is l % 16).
@@ -236,12 +249,28 @@ __device__ __forceinline__ int d_row<float>(int lane, int j) { return 4 * (lane 
 template <>
 __device__ __forceinline__ int d_row<double>(int lane, int j) { return (lane >> 4) + 4 * j; }
 
-template <typename T, int D, int P>
+// LO: FP32 only. The operator is split as A = A_hi + A_lo with A_hi = fl32(A) and A_lo the
+// residual rounded to bf16; after every 8 k-steps one 16x16x32 bf16 MFMA adds A_lo times the same
+// 8 B values (packed to bf16), so the systematic rounding of the operator drops from 2^-24 to
+// about 2^-33 relative at ~1/16 extra MFMA issue.
+// CS: FP32 only, compensated solution. U holds fl32(u) and Ulo a bf16 low word with u = U + Ulo
+// to about 2^-33 relative. The accumulator carries only the increment (stage 0 writes dt/6 k1
+// instead of u + dt/6 k1), and stage 3 adds it to (U, Ulo) with an exact two-sum, so the
+// per-step rounding of u (2^-24, accumulating over the run) is removed; stage inputs u + c k
+// are still formed from U alone, which costs only a non-accumulating 2^-24 perturbation.
+__device__ __forceinline__ float bf16_bits_to_f32(uint32_t b) { return __uint_as_float(b << 16); }
+__device__ __forceinline__ uint32_t f32_to_bf16_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;  // round to nearest even (finite inputs)
+}
+
+template <typename T, int D, int P, bool LO, bool CS>
 __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
+  static_assert(!CS || sizeof(T) == 4, "the compensated state is an FP32 form");
   using S = Sz<D, P>;
   using M = MfmaShape<D, P>;
   constexpr int NP = S::NP, NF = S::NF, NFP = S::NFP, NPERM = S::NPERM, NFF = M::NFF;
-  constexpr int NG = M::NG, NFK = M::NFK, KS = M::KS, MB = M::MB, TE = M::TE, RB = M::RB;
+  constexpr int NG = M::NG, NFK = M::NFK, KS = M::KS, MB = M::MB, TE = M::TE, RB = M::RB, NLO = M::NLO;
   constexpr int VW = 16 / sizeof(T);                        // values per 16-byte vector
   typedef T VT __attribute__((ext_vector_type(VW)));
   constexpr int PV = TE * NP / VW;                          // vectors per field plane of a tile
@@ -258,6 +287,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
   __shared__ int sFmask[NF * NFP];
   constexpr bool kALds = MB * KS * 64 * sizeof(T) <= 32768;  // fragments staged in LDS when they fit
   __shared__ T sA[kALds ? MB * KS * 64 : 1];
+  __shared__ bf16x8 sAlo[LO ? MB * NLO * 64 : 1];
   T* const sR = &sB[0][0];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -267,6 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
   T* __restrict__ Vout = (T*)a.Vout;
   T* __restrict__ U = (T*)a.U;
   T* __restrict__ Acc = (T*)a.A;
+  uint16_t* __restrict__ Ulo = (uint16_t*)a.Ulo;
   const T* __restrict__ Jinv = (const T*)a.Jinv;
   const T* __restrict__ Fg = (const T*)a.fgeom;
   const T* __restrict__ Af = (const T*)a.Amf;
@@ -279,6 +310,8 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
   if (kALds)
     for (int i = tid; i < MB * KS * 64; i += kBlock) sA[i] = Af[i];
   const T* __restrict__ pA = (kALds ? (const T*)sA : Af) + lane;
+  if (LO)
+    for (int i = tid; i < MB * NLO * 64; i += kBlock) sAlo[i] = ((const bf16x8*)a.Alo)[i];
 
   // Per-thread offsets that do not depend on the tile (every tile is full: the element count is
   // padded to a multiple of TE and padding elements carry zero state and geometry).
@@ -364,12 +397,14 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
         for (int c = 0; c < 4; ++c) sN[idx][c] = pfg[t][c];
     }
     VT pu[NE], pa[NE];
+    uint2 plo[CS ? NE : 1];
     if (stage >= 1 && stage <= 3) {
 #pragma unroll
       for (int t = 0; t < NE; ++t)
         if (tid + t * kBlock < 4 * PV) {
           const int id = voff[t] + e0 * NP;
-          if (stage != 3) pu[t] = *(const VT*)(U + id);
+          if (stage != 3 || CS) pu[t] = *(const VT*)(U + id);
+          if (CS && stage == 3) plo[CS ? t : 0] = *(const uint2*)(Ulo + id);
           pa[t] = *(const VT*)(Acc + id);
         }
     }
@@ -433,6 +468,21 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
     V acc[MB];
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) acc[mb] = V{0, 0, 0, 0};
+    bf16x8 blo;
+    auto step = [&](const int s, const T b) {
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) acc[mb] = Mfma<T>::mma(pA[(mb * KS + s) * 64], b, acc[mb]);
+      if constexpr (LO) {
+        blo[s % 8] = (__bf16)b;
+        if (s % 8 == 7 || s == KS - 1) {
+          if (s % 8 != 7)
+#pragma unroll
+            for (int i = s % 8 + 1; i < 8; ++i) blo[i] = (__bf16)0.0f;
+#pragma unroll
+          for (int mb = 0; mb < MB; ++mb) acc[mb] = mma_bf16(sAlo[(mb * NLO + s / 8) * 64 + lane], blo, acc[mb]);
+        }
+      }
+    };
 #pragma unroll
     for (int m = 0; m < NG; ++m) {
       const int n = 4 * m + g;
@@ -444,20 +494,12 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
         vz = sU[3][el * NP + n];
       }
 #pragma unroll
-      for (int r = 0; r < D; ++r) {
-        const T b = alpha[r] * p + beta[r][0] * vx + beta[r][1] * vy + beta[r][2] * vz;
-        const int s = m * D + r;
-#pragma unroll
-        for (int mb = 0; mb < MB; ++mb) acc[mb] = Mfma<T>::mma(pA[(mb * KS + s) * 64], b, acc[mb]);
-      }
+      for (int r = 0; r < D; ++r) step(m * D + r, alpha[r] * p + beta[r][0] * vx + beta[r][1] * vy + beta[r][2] * vz);
     }
 #pragma unroll
     for (int f = 0; f < NFK; ++f) {
       const int jk = 4 * f + g;
-      const T b = jk < NFF ? sB[q][el * RB + jk] : (T)0;
-      const int s = D * NG + f;
-#pragma unroll
-      for (int mb = 0; mb < MB; ++mb) acc[mb] = Mfma<T>::mma(pA[(mb * KS + s) * 64], b, acc[mb]);
+      step(D * NG + f, jk < NFF ? sB[q][el * RB + jk] : (T)0);
     }
     __syncthreads();  // face fluxes consumed; sB now receives the results as [4][TE*NP]
 #pragma unroll
@@ -478,23 +520,52 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
       switch (stage) {
         case 0: {
           const VT u = *(const VT*)(&sU[0][0] + loff[t]);
-          *(VT*)(Acc + id) = u + kq * s6;
+          *(VT*)(Acc + id) = CS ? kq * s6 : u + kq * s6;
           *(VT*)(Vout + id) = u + kq * s2;
         } break;
         case 1: *(VT*)(Acc + id) = pa[t] + kq * s3; *(VT*)(Vout + id) = pu[t] + kq * s2; break;
         case 2: *(VT*)(Acc + id) = pa[t] + kq * s3; *(VT*)(Vout + id) = pu[t] + kq; break;
-        case 3: *(VT*)(U + id) = pa[t] + kq * s6; break;
+        case 3:
+          if constexpr (CS) {
+            const VT d = pa[t] + kq * s6;  // dt/6 (k1 + 2 k2 + 2 k3 + k4)
+            const uint2 lw = plo[CS ? t : 0];
+            const uint32_t lb[4] = {lw.x & 0xffffu, lw.x >> 16, lw.y & 0xffffu, lw.y >> 16};
+            VT hi;
+            uint32_t ob[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const float u = pu[t][i], x = d[i];
+              const float sum = u + x, bv = sum - u;
+              const float err = (u - (sum - bv)) + (x - bv);  // two-sum: u + x = sum + err exactly
+              const float lo = bf16_bits_to_f32(lb[i]) + err;
+              const float h2 = sum + lo;
+              hi[i] = h2;
+              ob[i] = f32_to_bf16_bits(lo - (h2 - sum));
+            }
+            *(VT*)(U + id) = hi;
+            *(uint2*)(Ulo + id) = make_uint2(ob[0] | (ob[1] << 16), ob[2] | (ob[3] << 16));
+          } else {
+            *(VT*)(U + id) = pa[t] + kq * s6;
+          }
+          break;
         case 4: *(VT*)(Vout + id) = kq; break;
-        default: *(VT*)(Vout + id) = *(const VT*)(&sU[0][0] + loff[t]) + kq; break;
+        default:
+          *(VT*)(Vout + id) = *(const VT*)(&sU[0][0] + loff[t]) + kq;
+          if (CS) *(uint2*)(Ulo + id) = make_uint2(0, 0);  // Vout becomes U: an Euler step drops the low word
+          break;
       }
     }
   }
 }
 
 template <typename T>
-__global__ void k_source(T* __restrict__ U, const int* __restrict__ idx, int64_t n, T value) {
+__global__ void k_source(T* __restrict__ U, uint16_t* __restrict__ Ulo, const int* __restrict__ idx, int64_t n,
+                         T value) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) U[idx[i]] = value;
+  if (i < n) {
+    U[idx[i]] = value;
+    if (Ulo) Ulo[idx[i]] = 0;
+  }
 }
 
 template <typename T>
@@ -505,13 +576,17 @@ __global__ void k_finite(const T* __restrict__ x, int64_t n, int* __restrict__ f
   if (__any(bad) && (threadIdx.x % 64) == 0) atomicOr(flag, 1);
 }
 
-__global__ void k_d2f(const double* __restrict__ s, float* __restrict__ d, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    d[i] = (float)s[i];
+// FP64 -> FP32 (+ bf16 low word of the remainder when lo is given) and back.
+__global__ void k_d2f(const double* __restrict__ s, float* __restrict__ d, uint16_t* __restrict__ lo, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float h = (float)s[i];
+    d[i] = h;
+    if (lo) lo[i] = (uint16_t)f32_to_bf16_bits((float)(s[i] - (double)h));
+  }
 }
-__global__ void k_f2d(const float* __restrict__ s, double* __restrict__ d, int64_t n) {
+__global__ void k_f2d(const float* __restrict__ s, const uint16_t* __restrict__ lo, double* __restrict__ d, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    d[i] = (double)s[i];
+    d[i] = (double)s[i] + (lo ? (double)bf16_bits_to_f32(lo[i]) : 0.0);
 }
 
 int num_cus() {
@@ -542,12 +617,13 @@ void launch_lds(StageArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((k_stage_lds<T, D, P>), dim3(8 * bpx), dim3(kBlock), 0, s, a);
 }
 
-template <typename T, int D, int P>
+template <typename T, int D, int P, bool LO, bool CS>
 void launch_mfma(StageArgs& a, hipStream_t s) {
   constexpr int TE = MfmaShape<D, P>::TE;
   static int occ = 0;
   if (!occ) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stage_mfma<T, D, P>, kBlock, 0) != hipSuccess || occ < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stage_mfma<T, D, P, LO, CS>, kBlock, 0) != hipSuccess ||
+        occ < 1)
       occ = 1;
   }
   a.tiles = a.Kpad / TE;
@@ -555,13 +631,18 @@ void launch_mfma(StageArgs& a, hipStream_t s) {
   int64_t per_xcd_cap = (int64_t)(num_cus() / 8) * occ;
   int bpx = (int)std::min<int64_t>(a.tiles_per_xcd, per_xcd_cap);
   if (bpx < 1) bpx = 1;
-  hipLaunchKernelGGL((k_stage_mfma<T, D, P>), dim3(8 * bpx), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((k_stage_mfma<T, D, P, LO, CS>), dim3(8 * bpx), dim3(kBlock), 0, s, a);
 }
 
 template <typename T, int D, int P>
 void launch_variant(Ctx& c, StageArgs& a, hipStream_t s) {
-  if (c.variant == kVariantLds) launch_lds<T, D, P>(a, s);
-  else launch_mfma<T, D, P>(a, s);
+  if (c.variant == kVariantLds) return launch_lds<T, D, P>(a, s);
+  if constexpr (std::is_same<T, float>::value) {
+    if (c.comp && c.oplo) return launch_mfma<T, D, P, true, true>(a, s);
+    if (c.comp) return launch_mfma<T, D, P, false, true>(a, s);
+    if (c.oplo) return launch_mfma<T, D, P, true, false>(a, s);
+  }
+  launch_mfma<T, D, P, false, false>(a, s);
 }
 
 template <typename T>
@@ -587,9 +668,11 @@ void launch_source(Ctx& c, const Ctx::Src& src, double value, hipStream_t st) {
   if (!src.n) return;
   unsigned nb = (unsigned)((src.n + 255) / 256);
   if (c.precision == DGK_FP64)
-    hipLaunchKernelGGL(k_source<double>, dim3(nb), dim3(256), 0, st, (double*)c.U, src.idx, src.n, value);
+    hipLaunchKernelGGL(k_source<double>, dim3(nb), dim3(256), 0, st, (double*)c.U, (uint16_t*)nullptr, src.idx,
+                       src.n, value);
   else
-    hipLaunchKernelGGL(k_source<float>, dim3(nb), dim3(256), 0, st, (float*)c.U, src.idx, src.n, (float)value);
+    hipLaunchKernelGGL(k_source<float>, dim3(nb), dim3(256), 0, st, (float*)c.U, (uint16_t*)(c.comp ? c.Ulo : nullptr),
+                       src.idx, src.n, (float)value);
   DGK_HIP(hipGetLastError());
 }
 
@@ -607,21 +690,21 @@ bool check_finite(Ctx& c) {
   return h == 0;
 }
 
-void convert_state(Ctx& c, const double* src, void* dst, int64_t n) {
+void convert_state(Ctx& c, const double* src, void* dst, int64_t n, uint16_t* lo) {
   if (c.precision == DGK_FP64) {
     DGK_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
     return;
   }
-  hipLaunchKernelGGL(k_d2f, dim3(1024), dim3(256), 0, c.stream, src, (float*)dst, n);
+  hipLaunchKernelGGL(k_d2f, dim3(1024), dim3(256), 0, c.stream, src, (float*)dst, lo, n);
   DGK_HIP(hipGetLastError());
 }
 
-void convert_state_back(Ctx& c, const void* src, double* dst, int64_t n) {
+void convert_state_back(Ctx& c, const void* src, double* dst, int64_t n, const uint16_t* lo) {
   if (c.precision == DGK_FP64) {
     DGK_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
     return;
   }
-  hipLaunchKernelGGL(k_f2d, dim3(1024), dim3(256), 0, c.stream, (const float*)src, dst, n);
+  hipLaunchKernelGGL(k_f2d, dim3(1024), dim3(256), 0, c.stream, (const float*)src, lo, dst, n);
   DGK_HIP(hipGetLastError());
 }
 

@@ -152,6 +152,10 @@ def gen_line(n, L, order, side_tags, names, config_text):
 FP64, FP32 = 0, 1
 SIGN_REFERENCE, SIGN_STABLE = 0, 1
 VARIANT_DEFAULT, VARIANT_LDS, VARIANT_MFMA = 0, 1, 2
+# fp32_flags: 0 = default FP32 form (compensated solution); PLAIN_STATE = plain FP32 solution;
+# OPERATOR_CORRECTION = add the bf16 operator-residual MFMA
+FP32_PLAIN_STATE, FP32_OPERATOR_CORRECTION = 1, 2
+DGK_OK, DGK_ERR_INVALID, DGK_ERR_UNSUPPORTED, DGK_ERR_MESH, DGK_ERR_DEVICE, DGK_ERR_NONFINITE = range(6)
 
 
 class _Mesh(ctypes.Structure):
@@ -165,7 +169,7 @@ class _Phys(ctypes.Structure):
 
 class _Opts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("flux_sign", ctypes.c_int), ("device", ctypes.c_int),
-                ("variant", ctypes.c_int), ("reserved", ctypes.c_int * 6)]
+                ("variant", ctypes.c_int), ("fp32_flags", ctypes.c_int), ("reserved", ctypes.c_int * 5)]
 
 
 class _Info(ctypes.Structure):
@@ -177,7 +181,9 @@ class _Info(ctypes.Structure):
 
 DEV_SYMBOLS = ("dgk_create", "dgk_destroy", "dgk_last_error", "dgk_info_get", "dgk_set_state", "dgk_get_state",
                "dgk_set_sources", "dgk_rk4", "dgk_euler", "dgk_rhs", "dgk_sync", "dgk_check_finite",
-               "dgk_connectivity", "dgk_rk4_timed", "dgk_variant_name")
+               "dgk_connectivity", "dgk_rk4_timed", "dgk_variant_name", "dgk_batch_create", "dgk_batch_destroy",
+               "dgk_batch_last_error", "dgk_batch_set_state", "dgk_batch_get_state", "dgk_batch_rk4",
+               "dgk_batch_sync", "dgk_batch_groups")
 
 
 def device():
@@ -205,6 +211,16 @@ def device():
         L.dgk_rk4_timed.argtypes = [vp, ctypes.c_int, ctypes.c_double, P(ctypes.c_double), P(ctypes.c_double)]
         L.dgk_variant_name.argtypes = [ctypes.c_int]
         L.dgk_variant_name.restype = ctypes.c_char_p
+        L.dgk_batch_create.argtypes = [ctypes.c_int, P(_Mesh), P(_Phys), P(ctypes.c_double), P(_Opts), P(vp)]
+        L.dgk_batch_destroy.argtypes = [vp]
+        L.dgk_batch_last_error.argtypes = [vp]
+        L.dgk_batch_last_error.restype = ctypes.c_char_p
+        L.dgk_batch_set_state.argtypes = [vp, ctypes.c_int, P(ctypes.c_double)]
+        L.dgk_batch_get_state.argtypes = [vp, ctypes.c_int, P(ctypes.c_double)]
+        L.dgk_batch_rk4.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.dgk_batch_sync.argtypes = [vp]
+        L.dgk_batch_groups.argtypes = [vp, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int64),
+                                       P(ctypes.c_int)]
         _dev = L
     return _dev
 
@@ -219,7 +235,7 @@ class Solver:
     """One libdgk context: the reference's Mesh + solver state on the GPU."""
 
     def __init__(self, mesh, c0, rho0, v0=(0.0, 0.0, 0.0), precision=FP64, flux_sign=SIGN_REFERENCE, device_id=0,
-                 variant=0):
+                 variant=0, fp32_flags=0):
         L = device()
         self.mesh = mesh
         self.K, self.Np = mesh.K, mesh.Np
@@ -229,7 +245,7 @@ class Solver:
         m = _Mesh(mesh.dim, mesh.order, mesh.K, _p(self._xyz), _p(self._tags, ctypes.c_int64),
                   _p(self._bc, ctypes.c_uint8))
         ph = _Phys(float(c0), float(rho0), (ctypes.c_double * 3)(*[float(x) for x in v0]))
-        op = _Opts(int(precision), int(flux_sign), int(device_id), int(variant))
+        op = _Opts(int(precision), int(flux_sign), int(device_id), int(variant), int(fp32_flags))
         h = ctypes.c_void_p()
         st = L.dgk_create(ctypes.byref(m), ctypes.byref(ph), ctypes.byref(op), ctypes.byref(h))
         if st:
@@ -294,3 +310,81 @@ class Solver:
         s, t = ctypes.c_double(), ctypes.c_double()
         self._check(device().dgk_rk4_timed(self.h, int(nsteps), float(dt), ctypes.byref(s), ctypes.byref(t)))
         return s.value, t.value
+
+
+SCHEDULE_PARALLEL, SCHEDULE_SEQUENTIAL = 0, 1
+
+
+class Batch:
+    """Many independent simulations advanced together (libdgk batch API).
+
+    Simulations sharing (dim, order, physics) are fused into one device context; each keeps
+    its own time step. States use the single-simulation layout u[4][K*Np].
+    """
+
+    def __init__(self, meshes, phys, dts, precision=FP64, flux_sign=SIGN_REFERENCE, device_id=0, variant=0,
+                 fp32_flags=0):
+        L = device()
+        n = len(meshes)
+        assert len(phys) == n and len(dts) == n
+        self.meshes = list(meshes)
+        self._keep = []
+        marr = (_Mesh * n)()
+        parr = (_Phys * n)()
+        for i, (m, ph) in enumerate(zip(meshes, phys)):
+            xyz = np.ascontiguousarray(m.node_xyz, dtype=np.float64)
+            tags = np.ascontiguousarray(m.node_tags, dtype=np.int64)
+            bc = np.ascontiguousarray(m.node_bc, dtype=np.uint8)
+            self._keep += [xyz, tags, bc]
+            marr[i] = _Mesh(m.dim, m.order, m.K, _p(xyz), _p(tags, ctypes.c_int64), _p(bc, ctypes.c_uint8))
+            v0 = ph.get("v0", (0.0, 0.0, 0.0))
+            parr[i] = _Phys(float(ph["c0"]), float(ph["rho0"]), (ctypes.c_double * 3)(*[float(x) for x in v0]))
+        dta = np.ascontiguousarray(dts, dtype=np.float64)
+        op = _Opts(int(precision), int(flux_sign), int(device_id), int(variant), int(fp32_flags))
+        h = ctypes.c_void_p()
+        st = L.dgk_batch_create(n, marr, parr, _p(dta), ctypes.byref(op), ctypes.byref(h))
+        self._keep = None  # the library copies the meshes
+        if st:
+            raise DGKError(st, L.dgk_batch_last_error(None).decode())
+        self.h = h
+
+    def _check(self, st):
+        if st:
+            raise DGKError(st, device().dgk_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            device().dgk_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_state(self, i, u):
+        m = self.meshes[i]
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert u.size == 4 * m.K * m.Np
+        self._check(device().dgk_batch_set_state(self.h, int(i), _p(u)))
+
+    def get_state(self, i):
+        m = self.meshes[i]
+        u = np.zeros((4, m.K * m.Np))
+        self._check(device().dgk_batch_get_state(self.h, int(i), _p(u)))
+        return u
+
+    def rk4(self, nsteps, schedule=SCHEDULE_PARALLEL):
+        self._check(device().dgk_batch_rk4(self.h, int(nsteps), int(schedule)))
+
+    def sync(self):
+        self._check(device().dgk_batch_sync(self.h))
+
+    def groups(self):
+        L = device()
+        g = ctypes.c_int()
+        self._check(L.dgk_batch_groups(self.h, ctypes.byref(g), None, None, None, None))
+        n = g.value
+        d, o, s = (np.zeros(n, np.int32) for _ in range(3))
+        e = np.zeros(n, np.int64)
+        self._check(L.dgk_batch_groups(self.h, ctypes.byref(g), _p(d, ctypes.c_int), _p(o, ctypes.c_int),
+                                       _p(e, ctypes.c_int64), _p(s, ctypes.c_int)))
+        return [dict(dim=int(d[i]), order=int(o[i]), elements=int(e[i]), sims=int(s[i])) for i in range(n)]

@@ -59,8 +59,16 @@ typedef struct {
   int flux_sign;   /* DGK_SIGN_REFERENCE or DGK_SIGN_STABLE */
   int device;      /* HIP device ordinal */
   int variant;     /* stage-kernel variant, 0 = autotuned/default (see dgk_variant_name) */
-  int reserved[6];
+  int fp32_flags;  /* DGK_FP32 only: DGK_FP32_* bits, 0 = the default FP32 form */
+  int reserved[5];
 } dgk_opts;
+
+/* fp32_flags bits. By default the FP32 form keeps the solution u as an FP32 value plus a bf16
+ * low word (compensated: the RK4 update u += dt/6 (k1+2k2+2k3+k4) is summed exactly), which
+ * removes the step-by-step accumulation of FP32 rounding in u. PLAIN_STATE turns that off
+ * (plain FP32 u). OPERATOR_CORRECTION adds a bf16 residual of the FP32-rounded operator
+ * matrices on MFMA (about 6% slower, small further gain). */
+enum { DGK_FP32_PLAIN_STATE = 1, DGK_FP32_OPERATOR_CORRECTION = 2 };
 
 typedef struct {
   int dim, order, Np, Nf, Nfp;
@@ -115,6 +123,27 @@ dgk_status dgk_connectivity(dgk_ctx* ctx, int32_t* nbr, int32_t* nface, int32_t*
 dgk_status dgk_rk4_timed(dgk_ctx* ctx, int nsteps, double dt, double* stage_ms, double* total_ms);
 
 const char* dgk_variant_name(int variant);
+
+/* ---- Batched ensembles -------------------------------------------------------------------
+ * Many independent simulations (any mix of dimensions, orders and sizes) advanced together.
+ * No reference counterpart: the reference runs one simulation per process. Simulations with the
+ * same (dim, order, physics, flux sign) share one device context built on their disjoint union;
+ * each simulation keeps its own time step (folded into its elements' geometry), so one stage
+ * launch advances all of them. dgk_batch_rk4 captures the groups' stage launches as parallel
+ * branches of one hipGraph. States use the single-simulation layout u[4][K_i*Np_i]. */
+typedef struct dgk_batch dgk_batch;
+dgk_status dgk_batch_create(int n, const dgk_mesh* meshes, const dgk_phys* phys, const double* dt,
+                            const dgk_opts* opts, dgk_batch** out);
+dgk_status dgk_batch_destroy(dgk_batch* b);
+const char* dgk_batch_last_error(const dgk_batch* b);
+dgk_status dgk_batch_set_state(dgk_batch* b, int sim, const double* u);
+dgk_status dgk_batch_get_state(dgk_batch* b, int sim, double* u);
+/* schedule: 0 = groups as parallel graph branches, 1 = groups in sequence on one stream */
+dgk_status dgk_batch_rk4(dgk_batch* b, int nsteps, int schedule);
+dgk_status dgk_batch_sync(dgk_batch* b);
+/* number of groups; per group g (arrays of length >= groups, any may be NULL):
+ * dim[g], order[g], elements[g] (sum of K_i), sims[g] */
+dgk_status dgk_batch_groups(const dgk_batch* b, int* groups, int* dim, int* order, int64_t* elements, int* sims);
 
 #ifdef __cplusplus
 }

@@ -174,8 +174,7 @@ def test_rhs_fp32(dim, p, variant):
     assert rel(s.rhs(u, 1e-4), o.rhs(u, 1e-4)) <= 2e-6
 
 
-@pytest.mark.parametrize("workload", ["square", "cube"])
-def test_fp32_error_bound(workload):
+def _fp32_run(workload, fp32_flags):
     if workload == "square":
         m, ph, dt, u0 = wl.square_2d()
         n = 1000
@@ -183,11 +182,59 @@ def test_fp32_error_bound(workload):
         m, ph, dt, u0 = wl.cube_3d(8, 3)
         n = 300
     o = ob.Oracle(m, ph["c0"], ph["rho0"], v0=ph["v0"], sign_mode=ob.SIGN_STABLE)
-    s = dgk.Solver(m, ph["c0"], ph["rho0"], v0=ph["v0"], precision=dgk.FP32, flux_sign=dgk.SIGN_STABLE)
+    s = dgk.Solver(m, ph["c0"], ph["rho0"], v0=ph["v0"], precision=dgk.FP32, flux_sign=dgk.SIGN_STABLE,
+                   fp32_flags=fp32_flags)
     s.set_state(u0)
     s.rk4(n, dt)
     g = s.get_state()
     c = o.step(u0, n, dt)
-    errs = [rel(g[q], c[q]) for q in range(m.dim + 1)]
-    print("fp32 rel-L2 per field", workload, errs)
+    return [rel(g[q], c[q]) for q in range(m.dim + 1)]
+
+
+FP32_FORMS = {"compensated": 0, "plain": dgk.FP32_PLAIN_STATE, "corrected": dgk.FP32_OPERATOR_CORRECTION}
+
+
+@pytest.mark.parametrize("workload", ["square", "cube"])
+@pytest.mark.parametrize("form", list(FP32_FORMS))
+def test_fp32_error_bound(workload, form):
+    errs = _fp32_run(workload, FP32_FORMS[form])
+    print("fp32 rel-L2 per field", workload, form, errs)
     assert max(errs) <= 1e-5
+
+
+def test_fp32_compensated_state_removes_accumulated_rounding():
+    plain = max(_fp32_run("square", dgk.FP32_PLAIN_STATE))
+    comp = max(_fp32_run("square", 0))
+    print(f"square 1000 steps: plain {plain:.3e} compensated {comp:.3e}")
+    assert comp < 0.5 * plain
+
+
+def test_fp32_compensated_state_round_trip():
+    m = mixed_mesh(3, 3)
+    s = dgk.Solver(m, 340.0, 1.225, precision=dgk.FP32)
+    u = np.random.default_rng(8).standard_normal((4, m.K * m.Np))
+    s.set_state(u)
+    back = s.get_state()
+    assert rel(back, u) <= 2.0 ** -30  # FP32 value plus bf16 low word
+    p = dgk.Solver(m, 340.0, 1.225, precision=dgk.FP32, fp32_flags=dgk.FP32_PLAIN_STATE)
+    p.set_state(u)
+    assert 2.0 ** -27 <= rel(p.get_state(), u) <= 2.0 ** -23
+
+
+def test_fp32_euler_and_sources_compensated():
+    m = mixed_mesh(2, 3, 8)
+    o = ob.Oracle(m, 340.0, 1.225, sign_mode=ob.SIGN_STABLE)
+    s = dgk.Solver(m, 340.0, 1.225, precision=dgk.FP32, flux_sign=dgk.SIGN_STABLE)
+    u0 = wl.gaussian(m, (0.5, 0.4, 0.0), 0.01)
+    src = np.array([[0, 0.3, 0.3, 0, 0.1, 0.5, 2000.0, 0.2, 4e-5]])
+    dt = 1e-6
+    ts = np.arange(60) * dt
+    s.set_sources(src)
+    s.set_state(u0)
+    s.rk4(60, dt, ts)
+    s.euler(10, dt, ts[:10] + 60 * dt)
+    s.rk4(30, dt, ts[:30] + 70 * dt)
+    ref = o.step(u0, 60, dt, t_steps=ts, sources=src)
+    ref = o.step(ref, 10, dt, t_steps=ts[:10] + 60 * dt, sources=src, euler=True)
+    ref = o.step(ref, 30, dt, t_steps=ts[:30] + 70 * dt, sources=src)
+    assert rel(s.get_state()[:3], ref[:3]) <= 1e-5
