@@ -1,23 +1,30 @@
-"""Benchmark: DOF-RK-stage updates/s of the DG hot path on one MI355X.
+"""Benchmark: DOF-RK-stage updates/s of the DG hot path on one MI355X, FP64 plain vs FP32 batched.
 
-Workload: the 3D order-3 drop-in case -- unit cube, 32^3 cells Kuhn-split into 196,608
-tetrahedra, interior nodes jittered by 10% of h (seed 2), 4 fields (p, vx, vy, vz), mean
-flow v0 = (50, 0, 0) m/s, absorbing boundaries, Gaussian pulse (sigma 0.04) at a seeded
-interior point, dt = 0.1 h_min / ((c0 + |v0|)(2p+1)) (CFL 0.1 on the DG node spacing). Synthetic data: the mesh and state are
-generated here; inputs are resident in HBM before the timed region.
+Three legs, each timed over K RK4 steps after W warmup steps, with inputs resident in HBM:
 
-One "step" is one classic RK4 step (4 fused stage launches). The FP64 drop-in path and
-the FP32 path run the same number of steps from the same state; the FP32 relative L2
-error per field against the FP64 result (which matches the CPU oracle to <= 1e-12) is
-reported with the rates.
+* fp64_plain -- the drop-in path on the 3D order-3 case: unit cube, 32^3 cells Kuhn-split into
+  196,608 tetrahedra, interior nodes jittered by 10% of h (seed 2), v0 = (50, 0, 0) m/s,
+  absorbing walls, Gaussian pulse (sigma 0.04) at a seeded point, dt = 0.1 h_min/((c0+|v0|)(2p+1)).
+* fp32_single -- the same case in the FP32 form (MFMA FP32, compensated FP32 + bf16 solution,
+  FP64 geometry rounded once); relative L2 per field against the FP64 leg after the same steps.
+* value: fp32 batched -- 256 independent simulations of uneven size (seeds 7+i): 128 2D
+  rectangles and 128 3D Kuhn boxes with sides U[1,3], K ~ U[5k, 80k], order ~ U{2..5}, walls
+  reflecting/absorbing with p = 1/2, Gaussian pulses with sigma ~ U[0.02, 0.08] x the shortest
+  side; every simulation with its own CFL-0.1 step. One batch: groups of equal (dim, order) as
+  parallel branches of one captured hipGraph of K steps. Its FP32 error is measured on 8
+  members (every 32nd) against FP64 runs of the same members for the same steps.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+A "step" is one RK4 step (4 stage launches) of every simulation in the leg. Counted unknowns
+are (dim+1) fields per node. Synthetic data: meshes and states are generated here.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--legs fp64,fp32,batch]
 """
 import argparse
 import json
 import os
 import sys
 import time
+from math import comb
 
 import numpy as np
 
@@ -29,35 +36,50 @@ import dgk  # noqa: E402
 import workloads as wl  # noqa: E402
 
 METRIC = "DOF-RK-stage updates/s: FP64 plain vs FP32-MFMA batched at stated L2 err"
+UNIT = "DOF-RK-stage updates/s"
 HBM_PEAK = 8.0e12
-FP64_PEAK = 78.6e12
-FP32_PEAK = 157.3e12
+FIELDS = ["p", "vx", "vy", "vz"]
 
 
-def algorithmic(m, info, word):
-    """Algorithmic bytes and flops per DOF-stage of the fused stage (4 fields, affine elements).
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
-    bytes: per RK stage the state moves on average 4 words per DOF (stage 1: read u, write acc and
-    the next stage input; stages 2-3: read stage input, u and acc, write acc and next input;
-    stage 4: read stage input and acc, write u), plus the neighbour trace of every interior face
-    read by each side (2 F_int Nfp / (K Np) words per DOF), plus per element the inverse Jacobian
-    (9 words) and per element face an int32 neighbour and an int32 permutation code.
-    flops: volume d*Np*Np + lift Nf*Nfp*Np multiply-adds per field and element, the contravariant
-    flux, the nodal face flux and the RK update.
+
+def element_sizes(dim, p):
+    """(Np, Nf, Nfp) of the order-p simplex."""
+    return comb(p + dim, dim), dim + 1, 1 if dim == 1 else comb(p + dim - 1, dim - 1)
+
+
+def algorithmic_bytes(dim, p, K, F, Fb, word, compensated):
+    """Algorithmic HBM bytes of one stage launch over K elements of one (dim, order).
+
+    Per counted DOF ((dim+1) fields per node): the stage state traffic averaged over the four
+    RK4 stages -- stage 1 reads u and writes the accumulator and the next stage input (3 words),
+    stages 2-3 read the stage input, u and the accumulator and write the accumulator and the
+    next input (5), stage 4 reads the stage input and the accumulator and writes u (3; 5 in the
+    compensated FP32 form, which also reads u and reads/writes the 2-byte low word) -- plus the
+    neighbour trace of every interior face read by each side. Per element: J^-1 (dim^2 words);
+    per element face: unit normal and face scale (dim+1 words) and an int32 neighbour and int32
+    permutation code.
     """
-    d, Np, Nf, Nfp, K = m.dim, m.Np, info.Nf, info.Nfp, m.K
-    nEq = 4
-    F_int = info.num_faces - info.num_boundary_faces
-    words = 4 + 2 * F_int * Nfp / (K * Np)
-    geo = (9 * word + 8 * Nf) / (nEq * Np)
-    bytes_per = word * words + geo
-    flops_per = 2 * d * Np + 2 * Nf * Nfp + (2 * d * 3) + (2 * F_int + info.num_boundary_faces) * Nfp * 16 / (K * Np * nEq) + 4
-    return bytes_per, flops_per
+    Np, Nf, Nfp = element_sizes(dim, p)
+    nf = dim + 1
+    dof = nf * K * Np
+    state = (16 + (2 if compensated else 0)) / 4 * word  # bytes per DOF per stage
+    traces = 2 * (F - Fb) * Nfp * nf * word
+    geo = K * (dim * dim * word + Nf * ((dim + 1) * word + 8))
+    return dof * state + traces + geo
 
 
-def run_form(m, ph, dt, u0, precision, steps, warmup, variant=0, fp32_flags=0):
-    s = dgk.Solver(m, ph["c0"], ph["rho0"], v0=ph["v0"], precision=precision, flux_sign=dgk.SIGN_STABLE,
-                   variant=variant, fp32_flags=fp32_flags)
+def roofline(bytes_per_launch, stage_s, kernel):
+    ach = bytes_per_launch / stage_s
+    return {"bound": "hbm", "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK, 4), "traffic": None, "kernel": kernel,
+            "kernel_avg_us": round(stage_s * 1e6, 2), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+
+
+def single_leg(m, ph, dt, u0, precision, steps, warmup):
+    s = dgk.Solver(m, ph["c0"], ph["rho0"], v0=ph["v0"], precision=precision, flux_sign=dgk.SIGN_STABLE)
     s.set_state(u0)
     s.rk4(warmup, dt)
     s.sync()
@@ -65,43 +87,123 @@ def run_form(m, ph, dt, u0, precision, steps, warmup, variant=0, fp32_flags=0):
     s.rk4(steps, dt)
     s.sync()
     wall = time.perf_counter() - t0
-    # per-launch kernel time with HIP events on the context's stream (same work, same state path)
-    nk = max(10, min(steps, 200))
-    stage_ms, total_ms = s.rk4_timed(nk, dt)
+    nk = 50  # per-launch kernel time: HIP events around every stage launch on the context's stream
+    stage_ms, _ = s.rk4_timed(nk, dt)
     state = s.get_state()
-    finite = bool(np.isfinite(state).all())
-    return s, wall, stage_ms / (4 * nk), state, finite, steps + warmup + nk
+    info = s.info
+    s.close()
+    dof = (m.dim + 1) * m.K * m.Np
+    word = 8 if precision == dgk.FP64 else 4
+    b = algorithmic_bytes(m.dim, m.order, m.K, info.num_faces, info.num_boundary_faces, word, precision == dgk.FP32)
+    stage_s = stage_ms * 1e-3 / (4 * nk)
+    return dict(rate=dof * 4 * steps / wall, wall=wall, state=state, total_steps=warmup + steps + nk,
+                roofline=roofline(b, stage_s, "k_stage_mfma<%s,3,3>" % ("double" if word == 8 else "float")),
+                finite=bool(np.isfinite(state).all()))
 
 
-def cpu_baseline(m, ph, dt, u0):
+def batch_leg(steps, warmup, members_checked=8, n=256):
+    t0 = time.perf_counter()
+    meshes, phys, dts, pulses = wl.ensemble(n=n)
+    log(f"batch: generated {n} meshes in {time.perf_counter() - t0:.1f} s, "
+        f"K_total={sum(m.K for m in meshes)}, DOF={wl.dof_count(meshes)}")
+    t0 = time.perf_counter()
+    b = dgk.Batch(meshes, phys, dts, precision=dgk.FP32, flux_sign=dgk.SIGN_STABLE)
+    for i, (m, pu) in enumerate(zip(meshes, pulses)):
+        b.set_state(i, wl.ensemble_state(m, pu))
+    groups = b.groups()
+    log(f"batch: device setup and state upload {time.perf_counter() - t0:.1f} s, {len(groups)} groups")
+    b.rk4(warmup)
+    t0 = time.perf_counter()
+    b.prepare(steps)
+    log(f"batch: captured {steps} steps x {len(groups)} groups in {time.perf_counter() - t0:.1f} s")
+    b.sync()
+    t0 = time.perf_counter()
+    b.rk4(steps)
+    b.sync()
+    wall = time.perf_counter() - t0
+    dof = wl.dof_count(meshes)
+    nk = 10
+    gms = b.rk4_timed(nk)
+    total = warmup + steps + nk
+    gtab = []
+    for g, ms in zip(groups, gms):
+        Np = element_sizes(g["dim"], g["order"])[0]
+        bl = algorithmic_bytes(g["dim"], g["order"], g["elements"], g["faces"], g["boundary_faces"], 4, True)
+        st = ms * 1e-3 / (4 * nk)
+        gtab.append(dict(dim=g["dim"], order=g["order"], sims=g["sims"], elements=g["elements"],
+                         dof=(g["dim"] + 1) * g["elements"] * Np, stage_us=round(st * 1e6, 1), bytes=bl, stage_s=st))
+    tot = sum(x["stage_s"] for x in gtab)
+    for x in gtab:
+        x["share"] = round(x["stage_s"] / tot, 3)
+        x["GBps"] = round(x["bytes"] / x["stage_s"] / 1e9, 1)
+    dom = max(gtab, key=lambda x: x["stage_s"])
+    roof = roofline(dom["bytes"], dom["stage_s"],
+                    "k_stage_mfma<float,%d,%d> (largest batch group)" % (dom["dim"], dom["order"]))
+    roof["sum_of_group_stage_us"] = round(tot * 1e6, 1)
+    roof["all_groups_GBps"] = round(sum(x["bytes"] for x in gtab) / tot / 1e9, 1)
+    # FP32 error on a subset against FP64 runs of the same members
+    chk = list(range(0, n, n // members_checked))
+    g32 = {i: b.get_state(i) for i in chk}
+    b.close()
+    sub = dgk.Batch([meshes[i] for i in chk], [phys[i] for i in chk], [dts[i] for i in chk], precision=dgk.FP64,
+                    flux_sign=dgk.SIGN_STABLE)
+    for j, i in enumerate(chk):
+        sub.set_state(j, wl.ensemble_state(meshes[i], pulses[i]))
+    sub.rk4(total)
+    errs = {f: 0.0 for f in FIELDS}
+    finite = True
+    for j, i in enumerate(chk):
+        r = sub.get_state(j)
+        finite &= bool(np.isfinite(g32[i]).all())
+        for q in range(meshes[i].dim + 1):
+            errs[FIELDS[q]] = max(errs[FIELDS[q]], float(np.linalg.norm(g32[i][q] - r[q]) / np.linalg.norm(r[q])))
+    sub.close()
+    for x in gtab:
+        del x["stage_s"], x["bytes"]
+    return dict(rate=dof * 4 * steps / wall, wall=wall, dof=dof, elements=sum(m.K for m in meshes), groups=gtab,
+                roofline=roof, errs=errs, checked=chk, total_steps=total, finite=finite,
+                sample=(meshes, phys, dts, pulses))
+
+
+def cpu_baseline(meshes, phys, dts, pulses, members):
+    """Literal (reference loop structure) oracle on a bounded sample: one RK4 step of each member."""
     import oracle_binding as ob
 
     threads = os.cpu_count() or 1
-    o = ob.Oracle(m, ph["c0"], ph["rho0"], v0=ph["v0"], sign_mode=ob.SIGN_STABLE, nthreads=threads,
-                  factorized_only=True)
-    t0 = time.perf_counter()
-    o.step(u0, 1, dt)
-    sec = time.perf_counter() - t0
-    dof = 4 * m.K * m.Np
-    return {"value": dof * 4 / sec, "unit": "DOF-RK-stage updates/s", "cores": threads, "kind": "port",
-            "sample": f"1 RK4 step of the full 196,608-tet workload with the factorized OpenMP oracle ({sec:.1f} s)"}
+    t_lit = t_fac = 0.0
+    dof_total = 0
+    for i in members:
+        m, ph = meshes[i], phys[i]
+        u = wl.ensemble_state(m, pulses[i])
+        o = ob.Oracle(m, ph["c0"], ph["rho0"], v0=ph["v0"], sign_mode=ob.SIGN_STABLE, nthreads=threads)
+        t0 = time.perf_counter()
+        o.step(u, 1, dts[i], mode=ob.LITERAL)
+        t_lit += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        o.step(u, 1, dts[i], mode=ob.FACTORIZED)
+        t_fac += time.perf_counter() - t0
+        del o
+        dof_total += (m.dim + 1) * m.K * m.Np
+    desc = ", ".join(f"member {i} ({meshes[i].dim}D p{meshes[i].order} K={meshes[i].K})" for i in members)
+    return {"value": dof_total * 4 / t_lit, "unit": UNIT, "cores": threads, "kind": "port",
+            "sample": f"1 RK4 step each of {desc} with the oracle's literal path (the reference's loop "
+                      f"structure and OpenMP layout), {t_lit:.1f} s",
+            "factorized_value": dof_total * 4 / t_fac}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp64,fp32", help="comma list of fp64,fp32")
-    ap.add_argument("--variant", type=int, default=0, help="stage-kernel variant (0 = default)")
-    ap.add_argument("--fp32-flags", type=int, default=0, help="FP32 form switches (dgk.FP32_* bits, 0 = full form)")
+    ap.add_argument("--legs", default="fp64,fp32,batch", help="comma list of fp64,fp32,batch")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="first run a plain device copy of MB MiB (reference dispatch for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is in DESIGN.md)",
-              file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is documented, "
+              "not built)", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         import torch
@@ -111,75 +213,65 @@ def main():
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-        print(f"[bench] copy calibration: {args.copy_calibration} MiB read + {args.copy_calibration} MiB written",
-              file=sys.stderr, flush=True)
+        log(f"copy calibration: {args.copy_calibration} MiB read + {args.copy_calibration} MiB written")
         del src, dst
-    m, ph, dt, u0 = wl.cube_3d()
-    dof = 4 * m.K * m.Np
+    legs = args.legs.split(",")
     res = {}
-    forms = [f for f in (("fp64", dgk.FP64), ("fp32", dgk.FP32)) if f[0] in args.forms.split(",")]
-    for name, prec in forms:
-        print(f"[bench] {name}: {args.warmup} warmup + {args.steps} timed RK4 steps, K={m.K}", file=sys.stderr,
-              flush=True)
-        s, wall, kern_ms, state, finite, nsteps_total = run_form(m, ph, dt, u0, prec, args.steps, args.warmup,
-                                                                 args.variant, args.fp32_flags)
-        info = s.info
-        rate = dof * 4 * args.steps / wall
-        word = 8 if prec == dgk.FP64 else 4
-        bpd, fpd = algorithmic(m, info, word)
-        kern_rate = dof / (kern_ms * 1e-3)
-        ach_bw = bpd * dof / (kern_ms * 1e-3)
-        res[name] = dict(rate=rate, wall=wall, kern_ms=kern_ms, state=state, finite=finite, bpd=bpd, fpd=fpd,
-                         ach_bw=ach_bw, kern_rate=kern_rate, variant=dgk.device().dgk_variant_name(info.variant).decode(),
-                         steps_total=nsteps_total)
-        s.close()
-        print(f"[bench] {name}: {rate:.4g} DOF-stage/s wall, stage kernel {kern_ms * 1e3:.1f} us "
-              f"({ach_bw / 1e9:.0f} GB/s algorithmic)", file=sys.stderr, flush=True)
-    if len(res) < 2:
-        for k, r in res.items():
-            print(json.dumps({"form": k, "rate": r["rate"], "kernel_avg_us": r["kern_ms"] * 1e3}))
-        return
-    g64, g32 = res["fp64"]["state"], res["fp32"]["state"]
-    names = ["p", "vx", "vy", "vz"]
-    errs = {names[q]: float(np.linalg.norm(g32[q] - g64[q]) / max(np.linalg.norm(g64[q]), 1e-300)) for q in range(4)}
-    r32, r64 = res["fp32"], res["fp64"]
-
-    def roof(r):
-        return {"bound": "hbm", "achieved": round(r["ach_bw"] / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": round(r["ach_bw"] / HBM_PEAK, 4), "traffic": None,
-                "kernel": ("k_stage_lds" if r["variant"] == "lds-tile" else "k_stage_mfma") + f" ({r['variant']})",
-                "kernel_avg_us": round(r["kern_ms"] * 1e3, 2),
-                "algorithmic_bytes_per_dof_stage": round(r["bpd"], 2),
-                "flop_per_dof_stage": round(r["fpd"], 1)}
-
-    out = {
-        "metric": METRIC,
-        "value": r32["rate"],
-        "unit": "DOF-RK-stage updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": r32["wall"] * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32",
-        "data": "synthetic",
-        "config": {"workload": "cube3d-p3: unit cube, 32^3 Kuhn cells (196,608 tets), 10% jitter, order 3, "
-                               "4 fields, v0=(50,0,0), absorbing walls, Gaussian pulse",
-                   "elements": m.K, "order": 3, "dofs": dof, "dt": dt, "flux_sign": "stable",
-                   "form": "fp32 MFMA arithmetic, fp32 solution with bf16 low word (compensated), fp64 geometry "
-                           "rounded once", "fp32_flags": args.fp32_flags},
-        "fp32_rel_l2_vs_fp64": {"after_steps": r32["steps_total"], **errs},
-        "fp64_plain": {"value": r64["rate"], "ms_per_step": r64["wall"] * 1e3 / args.steps, "dtype": "fp64",
-                       "roofline": roof(r64), "finite": r64["finite"]},
-        "fp32_over_fp64": r32["rate"] / r64["rate"],
-        "roofline": roof(r32),
-        "cpu_baseline": None,
-    }
-    if not args.no_cpu:
-        print("[bench] cpu baseline (oracle, factorized, all host threads)", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(m, ph, dt, u0)
+    if "fp64" in legs or "fp32" in legs:
+        m, ph, dt, u0 = wl.cube_3d()
+        for name, prec in (("fp64", dgk.FP64), ("fp32", dgk.FP32)):
+            if name not in legs:
+                continue
+            log(f"{name} cube: {args.warmup} warmup + {args.steps} timed RK4 steps, K={m.K}")
+            res[name] = single_leg(m, ph, dt, u0, prec, args.steps, args.warmup)
+            log(f"{name} cube: {res[name]['rate']:.4g} DOF-stage/s, stage kernel "
+                f"{res[name]['roofline']['kernel_avg_us']} us ({res[name]['roofline']['achieved']} GB/s)")
+    if "batch" in legs:
+        log(f"batch: {args.warmup} warmup + {args.steps} timed RK4 steps")
+        res["batch"] = batch_leg(args.steps, args.warmup)
+        log(f"batch: {res['batch']['rate']:.4g} DOF-stage/s, {res['batch']['wall']:.1f} s")
+    out = {"metric": METRIC, "unit": UNIT, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "data": "synthetic"}
+    if "fp32" in res and "fp64" in res:
+        g64, g32 = res["fp64"]["state"], res["fp32"]["state"]
+        res["fp32"]["errs"] = {FIELDS[q]: float(np.linalg.norm(g32[q] - g64[q]) / np.linalg.norm(g64[q]))
+                               for q in range(4)}
+    cube_cfg = ("cube3d-p3: unit cube, 32^3 Kuhn cells (196,608 tets), 10% jitter, order 3, 4 fields, "
+                "v0=(50,0,0), absorbing walls, Gaussian pulse, flux sign stable")
+    if "batch" in res:
+        bt = res["batch"]
+        out.update(value=bt["rate"], ms_per_step=bt["wall"] * 1e3 / args.steps, dtype="fp32",
+                   config={"workload": "ensemble256: 256 uneven simulations (128 2D tri rectangles, 128 3D Kuhn "
+                                       "boxes, K~U[5k,80k], order~U{2..5}, mixed walls, own dt each), FP32 "
+                                       "compensated form, groups as branches of one captured hipGraph",
+                           "simulations": 256, "elements": bt["elements"], "dofs": bt["dof"],
+                           "flux_sign": "stable"},
+                   fp32_rel_l2_vs_fp64={"after_steps": bt["total_steps"], "members": bt["checked"], **bt["errs"]},
+                   roofline=bt["roofline"], groups=bt["groups"], finite=bt["finite"])
+    elif "fp32" in res:
+        r = res["fp32"]
+        out.update(value=r["rate"], ms_per_step=r["wall"] * 1e3 / args.steps, dtype="fp32",
+                   config={"workload": cube_cfg}, roofline=r["roofline"])
+    elif "fp64" in res:
+        r = res["fp64"]
+        out.update(value=r["rate"], ms_per_step=r["wall"] * 1e3 / args.steps, dtype="fp64",
+                   config={"workload": cube_cfg}, roofline=r["roofline"])
+    if "fp64" in res:
+        r = res["fp64"]
+        out["fp64_plain"] = {"value": r["rate"], "ms_per_step": r["wall"] * 1e3 / args.steps, "dtype": "fp64",
+                             "workload": cube_cfg, "roofline": r["roofline"], "finite": r["finite"]}
+        if "batch" in res:
+            out["fp32_batched_over_fp64_plain"] = res["batch"]["rate"] / r["rate"]
+    if "fp32" in res:
+        r = res["fp32"]
+        out["fp32_single"] = {"value": r["rate"], "ms_per_step": r["wall"] * 1e3 / args.steps, "dtype": "fp32",
+                              "workload": cube_cfg, "roofline": r["roofline"], "finite": r["finite"]}
+        if "errs" in r:
+            out["fp32_single"]["rel_l2_vs_fp64"] = {"after_steps": r["total_steps"], **r["errs"]}
+    out["cpu_baseline"] = None
+    if not args.no_cpu and "batch" in res:
+        log("cpu baseline: literal oracle on a sample of the ensemble, all host threads")
+        out["cpu_baseline"] = cpu_baseline(*res["batch"]["sample"], members=[0, 134])
     print(json.dumps(out))
 
 

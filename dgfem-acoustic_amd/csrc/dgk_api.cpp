@@ -785,35 +785,45 @@ void batch_enqueue(dgk_batch& B, int nsteps, int schedule) {
   }
 }
 
+// Capture (once) the graph of `chunk` steps of every group under `schedule`.
+void batch_graph(dgk_batch& B, int chunk, int schedule) {
+  using namespace dgk;
+  if (B.graph && B.graph_steps == chunk && B.graph_schedule == schedule) return;
+  if (B.graph) { DGK_HIP(hipGraphExecDestroy(B.graph)); B.graph = nullptr; }
+  hipGraph_t gr;
+  DGK_HIP(hipStreamBeginCapture(B.main, hipStreamCaptureModeGlobal));
+  try {
+    batch_enqueue(B, chunk, schedule);
+  } catch (...) {
+    hipGraph_t junk = nullptr;
+    if (hipStreamEndCapture(B.main, &junk) == hipSuccess && junk) (void)hipGraphDestroy(junk);
+    throw;
+  }
+  DGK_HIP(hipStreamEndCapture(B.main, &gr));
+  hipError_t e = hipGraphInstantiate(&B.graph, gr, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(gr);
+  DGK_HIP(e);
+  B.graph_steps = chunk;
+  B.graph_schedule = schedule;
+}
+
+constexpr int kMaxGraphSteps = 1000;
+
+void batch_check(int nsteps, int schedule) {
+  if (nsteps < 0 || schedule < 0 || schedule > 1) throw dgk::InvalidArg("bad batch step arguments");
+}
+
 void batch_rk4_impl(dgk_batch& B, int nsteps, int schedule) {
   using namespace dgk;
-  if (nsteps < 0 || schedule < 0 || schedule > 1) throw InvalidArg("bad batch step arguments");
+  batch_check(nsteps, schedule);
   for (auto& G : B.groups)  // the batch stream runs after anything queued on the contexts
     DGK_HIP(hipStreamSynchronize(G.ctx->stream));
   // Up to kMaxGraphSteps steps of every group are captured as one graph and replayed; a
   // capture is reused while the step count and schedule stay the same.
-  constexpr int kMaxGraphSteps = 1000;
   const int chunk = std::min(nsteps, kMaxGraphSteps);
   int done = 0;
   while (chunk > 0 && nsteps - done >= chunk) {
-    if (!B.graph || B.graph_steps != chunk || B.graph_schedule != schedule) {
-      if (B.graph) { DGK_HIP(hipGraphExecDestroy(B.graph)); B.graph = nullptr; }
-      hipGraph_t gr;
-      DGK_HIP(hipStreamBeginCapture(B.main, hipStreamCaptureModeGlobal));
-      try {
-        batch_enqueue(B, chunk, schedule);
-      } catch (...) {
-        hipGraph_t junk;
-        (void)hipStreamEndCapture(B.main, &junk);
-        throw;
-      }
-      DGK_HIP(hipStreamEndCapture(B.main, &gr));
-      hipError_t e = hipGraphInstantiate(&B.graph, gr, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(gr);
-      DGK_HIP(e);
-      B.graph_steps = chunk;
-      B.graph_schedule = schedule;
-    }
+    batch_graph(B, chunk, schedule);
     DGK_HIP(hipGraphLaunch(B.graph, B.main));
     done += chunk;
   }
@@ -892,6 +902,41 @@ dgk_status dgk_batch_rk4(dgk_batch* b, int nsteps, int schedule) {
   return bguard(b, [&] { batch_rk4_impl(*b, nsteps, schedule); });
 }
 
+dgk_status dgk_batch_prepare(dgk_batch* b, int nsteps, int schedule) {
+  if (!b) return bfail(nullptr, DGK_ERR_INVALID, "NULL batch");
+  return bguard(b, [&] {
+    batch_check(nsteps, schedule);
+    if (nsteps > 0) batch_graph(*b, std::min(nsteps, kMaxGraphSteps), schedule);
+  });
+}
+
+dgk_status dgk_batch_rk4_timed(dgk_batch* b, int nsteps, double* group_stage_ms) {
+  if (!b || nsteps < 1 || !group_stage_ms) return bfail(b, DGK_ERR_INVALID, "bad arguments");
+  return bguard(b, [&] {
+    DGK_HIP(hipStreamSynchronize(b->main));
+    for (size_t g = 0; g < b->groups.size(); ++g) {
+      dgk::Ctx& c = *b->groups[g].ctx;
+      std::vector<hipEvent_t> ev(8 * nsteps);
+      for (auto& e : ev) DGK_HIP(hipEventCreate(&e));
+      try {
+        for (int i = 0; i < nsteps; ++i) dgk::rk4_step(c, 1.0, c.stream, &ev[8 * i]);
+        DGK_HIP(hipEventSynchronize(ev.back()));
+        double sum = 0;
+        for (int i = 0; i < 4 * nsteps; ++i) {
+          float ms = 0;
+          DGK_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+          sum += ms;
+        }
+        group_stage_ms[g] = sum;
+      } catch (...) {
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        throw;
+      }
+      for (auto& e : ev) (void)hipEventDestroy(e);
+    }
+  });
+}
+
 dgk_status dgk_batch_sync(dgk_batch* b) {
   if (!b) return bfail(nullptr, DGK_ERR_INVALID, "NULL batch");
   return bguard(b, [&] {
@@ -900,14 +945,18 @@ dgk_status dgk_batch_sync(dgk_batch* b) {
   });
 }
 
-dgk_status dgk_batch_groups(const dgk_batch* b, int* groups, int* dim, int* order, int64_t* elements, int* sims) {
+dgk_status dgk_batch_groups(const dgk_batch* b, int* groups, int* dim, int* order, int64_t* elements, int* sims,
+                            int64_t* faces, int64_t* boundary_faces) {
   if (!b) return bfail(nullptr, DGK_ERR_INVALID, "NULL batch");
   if (groups) *groups = (int)b->groups.size();
   for (size_t g = 0; g < b->groups.size(); ++g) {
-    if (dim) dim[g] = b->groups[g].ctx->dim;
-    if (order) order[g] = b->groups[g].ctx->p;
-    if (elements) elements[g] = b->groups[g].ctx->K;
+    const dgk::Ctx& c = *b->groups[g].ctx;
+    if (dim) dim[g] = c.dim;
+    if (order) order[g] = c.p;
+    if (elements) elements[g] = c.K;
     if (sims) sims[g] = (int)b->groups[g].sims.size();
+    if (faces) faces[g] = c.F;
+    if (boundary_faces) boundary_faces[g] = c.Fb;
   }
   return DGK_OK;
 }

@@ -285,7 +285,11 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
   __shared__ T sG[TE][9];
   __shared__ int sVmap[NF * NPERM * NFP];
   __shared__ int sFmask[NF * NFP];
-  constexpr bool kALds = MB * KS * 64 * sizeof(T) <= 32768;  // fragments staged in LDS when they fit
+  // Operator fragments are staged in LDS whenever the whole workgroup footprint fits the 160 KiB
+  // of a gfx950 CU (fewer resident workgroups, but no per-k-step global loads).
+  constexpr size_t kLdsRest = sizeof(T) * (4 * TE * NP + 4 * TE * RB + TE * NF * 4 + TE * 9) +
+                              sizeof(int) * (NF * NPERM * NFP + NF * NFP) + (LO ? 16 * MB * NLO * 64 : 0);
+  constexpr bool kALds = MB * KS * 64 * sizeof(T) + kLdsRest <= 160 * 1024 - 512;
   __shared__ T sA[kALds ? MB * KS * 64 : 1];
   __shared__ bf16x8 sAlo[LO ? MB * NLO * 64 : 1];
   T* const sR = &sB[0][0];

@@ -115,13 +115,19 @@ class Mesh:
             self.bc_groups.append((int(tags[i]), int(refl[i]), nodes))
 
     def write(self, path):
+        if self.h is None:
+            raise RuntimeError("the host mesh model was released")
         if host().dgh_write(self.h, path.encode()):
             raise RuntimeError(host().dgh_last_error().decode())
 
-    def __del__(self):
+    def release(self):
+        """Free the host-side mesh model (the element arrays stay); write() is then unavailable."""
         if getattr(self, "h", None) is not None:
             host().dgh_free(self.h)
             self.h = None
+
+    def __del__(self):
+        self.release()
 
 
 def read_mesh(path, config_text):
@@ -183,7 +189,7 @@ DEV_SYMBOLS = ("dgk_create", "dgk_destroy", "dgk_last_error", "dgk_info_get", "d
                "dgk_set_sources", "dgk_rk4", "dgk_euler", "dgk_rhs", "dgk_sync", "dgk_check_finite",
                "dgk_connectivity", "dgk_rk4_timed", "dgk_variant_name", "dgk_batch_create", "dgk_batch_destroy",
                "dgk_batch_last_error", "dgk_batch_set_state", "dgk_batch_get_state", "dgk_batch_rk4",
-               "dgk_batch_sync", "dgk_batch_groups")
+               "dgk_batch_sync", "dgk_batch_groups", "dgk_batch_prepare", "dgk_batch_rk4_timed")
 
 
 def device():
@@ -219,8 +225,10 @@ def device():
         L.dgk_batch_get_state.argtypes = [vp, ctypes.c_int, P(ctypes.c_double)]
         L.dgk_batch_rk4.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.dgk_batch_sync.argtypes = [vp]
+        L.dgk_batch_prepare.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.dgk_batch_rk4_timed.argtypes = [vp, ctypes.c_int, P(ctypes.c_double)]
         L.dgk_batch_groups.argtypes = [vp, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int64),
-                                       P(ctypes.c_int)]
+                                       P(ctypes.c_int), P(ctypes.c_int64), P(ctypes.c_int64)]
         _dev = L
     return _dev
 
@@ -378,13 +386,26 @@ class Batch:
     def sync(self):
         self._check(device().dgk_batch_sync(self.h))
 
+    def prepare(self, nsteps, schedule=SCHEDULE_PARALLEL):
+        """Capture the graph for rk4(nsteps, schedule) ahead of time."""
+        self._check(device().dgk_batch_prepare(self.h, int(nsteps), int(schedule)))
+
+    def rk4_timed(self, nsteps):
+        """Run nsteps per group with event-timed stage launches; summed stage ms per group."""
+        n = len(self.groups())
+        out = np.zeros(n)
+        self._check(device().dgk_batch_rk4_timed(self.h, int(nsteps), _p(out)))
+        return out
+
     def groups(self):
         L = device()
         g = ctypes.c_int()
-        self._check(L.dgk_batch_groups(self.h, ctypes.byref(g), None, None, None, None))
+        self._check(L.dgk_batch_groups(self.h, ctypes.byref(g), None, None, None, None, None, None))
         n = g.value
         d, o, s = (np.zeros(n, np.int32) for _ in range(3))
-        e = np.zeros(n, np.int64)
+        e, f, fb = (np.zeros(n, np.int64) for _ in range(3))
         self._check(L.dgk_batch_groups(self.h, ctypes.byref(g), _p(d, ctypes.c_int), _p(o, ctypes.c_int),
-                                       _p(e, ctypes.c_int64), _p(s, ctypes.c_int)))
-        return [dict(dim=int(d[i]), order=int(o[i]), elements=int(e[i]), sims=int(s[i])) for i in range(n)]
+                                       _p(e, ctypes.c_int64), _p(s, ctypes.c_int), _p(f, ctypes.c_int64),
+                                       _p(fb, ctypes.c_int64)))
+        return [dict(dim=int(d[i]), order=int(o[i]), elements=int(e[i]), sims=int(s[i]), faces=int(f[i]),
+                     boundary_faces=int(fb[i])) for i in range(n)]

@@ -140,10 +140,16 @@ dgk_status dgk_batch_set_state(dgk_batch* b, int sim, const double* u);
 dgk_status dgk_batch_get_state(dgk_batch* b, int sim, double* u);
 /* schedule: 0 = groups as parallel graph branches, 1 = groups in sequence on one stream */
 dgk_status dgk_batch_rk4(dgk_batch* b, int nsteps, int schedule);
+/* capture (and keep) the graph that dgk_batch_rk4(b, nsteps, schedule) will replay, without running it */
+dgk_status dgk_batch_prepare(dgk_batch* b, int nsteps, int schedule);
+/* nsteps RK4 steps group by group without a graph, with a HIP event pair around every stage
+ * launch; group_stage_ms[g] receives the summed stage-kernel time of group g */
+dgk_status dgk_batch_rk4_timed(dgk_batch* b, int nsteps, double* group_stage_ms);
 dgk_status dgk_batch_sync(dgk_batch* b);
-/* number of groups; per group g (arrays of length >= groups, any may be NULL):
- * dim[g], order[g], elements[g] (sum of K_i), sims[g] */
-dgk_status dgk_batch_groups(const dgk_batch* b, int* groups, int* dim, int* order, int64_t* elements, int* sims);
+/* number of groups; per group g (arrays of length >= groups, any may be NULL): dim[g], order[g],
+ * elements[g] (sum of K_i), sims[g], faces[g] (unique faces), boundary_faces[g] */
+dgk_status dgk_batch_groups(const dgk_batch* b, int* groups, int* dim, int* order, int64_t* elements, int* sims,
+                            int64_t* faces, int64_t* boundary_faces);
 
 #ifdef __cplusplus
 }

@@ -6,8 +6,11 @@ Time steps use a CFL number of 0.1 relative to the DG node spacing, dt = 0.1 h_m
 
 Sizes follow the benchmark configurations: a structured 2D square at order 3, a jittered
 unit cube with a Kuhn tetrahedral split at order 3, the 20x15x8 m room with a block
-obstacle at order 4, plus small variants of each for parity tests.
+obstacle at order 4, the 256-member ensemble of uneven 2D/3D simulations at orders 2-5,
+plus small variants of each for parity tests.
 """
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 import dgk
@@ -71,3 +74,54 @@ def room_3d(h=0.25, order=4, seed=4):
     dt = cfl_dt(h, 343.0, order)
     u0 = gaussian(m, center, 2 * sigma ** 2)
     return m, phys, dt, u0
+
+
+def ensemble_member(i, seed0=7, n=256, kmin=5000, kmax=80000, orders=(2, 3, 4, 5)):
+    """Member i of the uneven ensemble: the first half 2D rectangles (2 triangles per cell), the
+    rest 3D boxes (Kuhn split), sides U[1,3], K ~ U[kmin,kmax], order ~ U{orders}, each wall
+    reflecting or absorbing with probability 1/2, Gaussian pulse with sigma ~ U[0.02,0.08] times
+    the shortest side, centred at least 3 sigma from every wall. Seeds 7+i (MT19937)."""
+    rng = np.random.Generator(np.random.MT19937(seed0 + i))
+    dim = 2 if i < n // 2 else 3
+    K_target = int(rng.integers(kmin, kmax + 1))
+    p = int(orders[int(rng.integers(len(orders)))])
+    L = 1.0 + 2.0 * rng.random(dim)
+    walls = [1 if r < 0.5 else 2 for r in rng.random(2 * dim)]
+    sigma = (0.02 + 0.06 * rng.random()) * float(L.min())
+    center = [3 * sigma + (L[k] - 6 * sigma) * rng.random() for k in range(dim)] + [0.0] * (3 - dim)
+    names = "1:Wall;2:Abs"
+    text = conf("Wall=Reflecting\nAbs=Absorbing\n")
+    if dim == 2:
+        h = np.sqrt(2.0 * L[0] * L[1] / K_target)
+        nx, ny = max(1, int(round(L[0] / h))), max(1, int(round(L[1] / h)))
+        m = dgk.gen_rect(nx, ny, float(L[0]), float(L[1]), p, walls, names, text)
+        hmin = min(L[0] / nx, L[1] / ny)
+    else:
+        h = np.cbrt(6.0 * L[0] * L[1] * L[2] / K_target)
+        nx, ny, nz = (max(1, int(round(L[k] / h))) for k in range(3))
+        m = dgk.gen_box(nx, ny, nz, float(L[0]), float(L[1]), float(L[2]), p, 0.0, 0, walls, names, text)
+        hmin = min(L[0] / nx, L[1] / ny, L[2] / nz)
+    m.release()
+    phys = dict(c0=340.0, rho0=1.225, v0=(0.0, 0.0, 0.0))
+    dt = cfl_dt(hmin, 340.0, p)
+    return m, phys, dt, center, sigma
+
+
+def ensemble(n=256, seed0=7, kmin=5000, kmax=80000, orders=(2, 3, 4, 5), threads=16, members=None):
+    """The uneven ensemble (see ensemble_member), generated on a thread pool (the generators
+    release the GIL). Returns lists (meshes, phys, dts, pulses); ensemble_state(mesh, pulse)
+    builds a member's initial state."""
+    idx = list(range(n)) if members is None else list(members)
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        out = list(ex.map(lambda i: ensemble_member(i, seed0, n, kmin, kmax, orders), idx))
+    return [o[0] for o in out], [o[1] for o in out], [o[2] for o in out], [(o[3], o[4]) for o in out]
+
+
+def ensemble_state(mesh, pulse):
+    center, sigma = pulse
+    return gaussian(mesh, center, 2 * sigma ** 2)
+
+
+def dof_count(meshes):
+    """Counted unknowns: (dim+1) fields per node."""
+    return sum((m.dim + 1) * m.K * m.Np for m in meshes)
