@@ -10,7 +10,14 @@
 
 namespace dgk {
 
+void Ctx::free_receivers() {
+  for (void* p : {(void*)rec.base, rec.w, rec.out, (void*)rec.counter})
+    if (p) (void)hipFree(p);
+  rec = Rec{};
+}
+
 Ctx::~Ctx() {
+  free_receivers();
   if (graph) (void)hipGraphExecDestroy(graph);
   for (auto& s : sources)
     if (s.idx) (void)hipFree(s.idx);
@@ -369,6 +376,118 @@ void rk4_step(Ctx& c, double dt, hipStream_t s, hipEvent_t* ev) {
     launch_stage(c, a, s);
     if (ev) DGK_HIP(hipEventRecord(ev[2 * st + 1], s));
   }
+  launch_record(c, s);
+}
+
+// Locate point x in a mesh given by its element node coordinates (Gmsh order, vertices first):
+// the element whose smallest barycentric coordinate of x is largest. Returns the element and the
+// element's Lagrange basis at x (the pressure at x is sum_n w[n] p[e*Np + n]).
+int64_t locate_point(const RefElem& R, const double* node_xyz, int64_t K, const double* x, std::vector<double>& w) {
+  const int d = R.dim, Np = R.Np;
+  int64_t best = -1;
+  double best_score = -1e300, best_l[4] = {0, 0, 0, 0};
+  for (int64_t e = 0; e < K; ++e) {
+    const double* X = node_xyz + (size_t)e * Np * 3;
+    // bounding box prefilter with a tolerance of the box size
+    double lo[3], hi[3];
+    for (int c = 0; c < 3; ++c) {
+      lo[c] = hi[c] = X[c];
+      for (int v = 1; v <= d; ++v) {
+        lo[c] = std::min(lo[c], X[3 * v + c]);
+        hi[c] = std::max(hi[c], X[3 * v + c]);
+      }
+    }
+    bool out = false;
+    for (int c = 0; c < 3; ++c) {
+      const double tol = 1e-9 + 1e-6 * (hi[c] - lo[c]);
+      if (x[c] < lo[c] - tol || x[c] > hi[c] + tol) out = true;
+    }
+    if (out && best >= 0) continue;
+    // barycentric coordinates by the normal equations of x - v0 = sum_a l_a (v_a - v0)
+    double J[3][3] = {}, G[3][3] = {}, r[3] = {}, b[3] = {};
+    for (int a = 0; a < d; ++a)
+      for (int c = 0; c < 3; ++c) J[c][a] = X[3 * (a + 1) + c] - X[c];
+    for (int c = 0; c < 3; ++c) r[c] = x[c] - X[c];
+    for (int a = 0; a < d; ++a) {
+      for (int bb = 0; bb < d; ++bb)
+        for (int c = 0; c < 3; ++c) G[a][bb] += J[c][a] * J[c][bb];
+      for (int c = 0; c < 3; ++c) b[a] += J[c][a] * r[c];
+    }
+    std::vector<ld> Gi(d * d);
+    for (int a = 0; a < d; ++a)
+      for (int bb = 0; bb < d; ++bb) Gi[a * d + bb] = G[a][bb];
+    invert(Gi, d);
+    double l[4];
+    double sum = 0;
+    for (int a = 0; a < d; ++a) {
+      ld t = 0;
+      for (int bb = 0; bb < d; ++bb) t += Gi[a * d + bb] * b[bb];
+      l[a + 1] = (double)t;
+      sum += l[a + 1];
+    }
+    l[0] = 1 - sum;
+    double score = l[0];
+    for (int a = 1; a <= d; ++a) score = std::min(score, l[a]);
+    if (score > best_score) {
+      best_score = score;
+      best = e;
+      for (int a = 0; a <= d; ++a) best_l[a] = l[a];
+    }
+  }
+  if (best < 0 || best_score < -1e-6) throw InvalidArg("receiver point outside the mesh");
+  std::array<ld, 3> ref{0, 0, 0};
+  if (d == 1) ref[0] = best_l[1] - best_l[0];
+  else
+    for (int a = 0; a < d; ++a) ref[a] = best_l[a + 1];
+  std::vector<ld> phi(Np);
+  R.eval_basis(ref, phi.data(), nullptr);
+  w.assign(Np, 0);
+  for (int n = 0; n < Np; ++n) w[n] = (double)phi[n];
+  return best;
+}
+
+// Install receivers (element, weights) on the context, with room for cap recorded steps.
+void install_receivers(Ctx& c, const std::vector<int64_t>& elem, const std::vector<double>& w, int cap) {
+  DGK_HIP(hipStreamSynchronize(c.stream));
+  c.free_receivers();
+  if (c.graph) { DGK_HIP(hipGraphExecDestroy(c.graph)); c.graph = nullptr; }
+  const int n = (int)elem.size();
+  if (!n) return;
+  if (cap < 1) throw InvalidArg("receiver capacity must be positive");
+  std::vector<int> base(n);
+  for (int i = 0; i < n; ++i) base[i] = (int)(elem[i] * c.Np);
+  DGK_HIP(hipMalloc(&c.rec.base, n * sizeof(int)));
+  DGK_HIP(hipMalloc(&c.rec.w, (size_t)n * c.Np * c.tsize));
+  DGK_HIP(hipMalloc(&c.rec.out, (size_t)n * cap * c.tsize));
+  DGK_HIP(hipMalloc(&c.rec.counter, sizeof(int)));
+  c.rec.n = n;
+  c.rec.cap = cap;
+  DGK_HIP(hipMemcpy(c.rec.base, base.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  if (c.precision == DGK_FP64) {
+    DGK_HIP(hipMemcpy(c.rec.w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> wf(w.begin(), w.end());
+    DGK_HIP(hipMemcpy(c.rec.w, wf.data(), wf.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  DGK_HIP(hipMemset(c.rec.counter, 0, sizeof(int)));
+}
+
+// Copy the recorded rows (all receivers) to host doubles; returns the row count.
+int fetch_receivers(Ctx& c, std::vector<double>& out) {
+  int rows = 0;
+  if (!c.rec.n) { out.clear(); return 0; }
+  DGK_HIP(hipStreamSynchronize(c.stream));
+  DGK_HIP(hipMemcpy(&rows, c.rec.counter, sizeof(int), hipMemcpyDeviceToHost));
+  const size_t m = (size_t)rows * c.rec.n;
+  out.resize(m);
+  if (c.precision == DGK_FP64) {
+    DGK_HIP(hipMemcpy(out.data(), c.rec.out, m * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> f(m);
+    DGK_HIP(hipMemcpy(f.data(), c.rec.out, m * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < m; ++i) out[i] = f[i];
+  }
+  return rows;
 }
 
 namespace {
@@ -436,6 +555,7 @@ void euler_impl(Ctx& c, int nsteps, double dt, const double* ts) {
     launch_stage(c, a, c.stream);
     std::swap(c.U, c.V0);  // graphs capture buffer addresses; drop any cached one
     if (c.graph) { DGK_HIP(hipGraphExecDestroy(c.graph)); c.graph = nullptr; }
+    launch_record(c, c.stream);
   }
 }
 
@@ -556,6 +676,33 @@ dgk_status dgk_set_sources(dgk_ctx* c, int n, const double* src, const double* n
   });
 }
 
+dgk_status dgk_set_receivers(dgk_ctx* c, int n, const double* xyz, const double* node_xyz, int capacity) {
+  if (!c || n < 0 || (n > 0 && (!xyz || !node_xyz))) return fail(c, DGK_ERR_INVALID, "bad receiver arguments");
+  return guarded(c, [&] {
+    RefElem R = make_ref_elem(c->dim, c->p);
+    std::vector<int64_t> elem(n);
+    std::vector<double> w, wi;
+    for (int i = 0; i < n; ++i) {
+      elem[i] = locate_point(R, node_xyz, c->K, xyz + 3 * i, wi);
+      w.insert(w.end(), wi.begin(), wi.end());
+    }
+    install_receivers(*c, elem, w, capacity);
+  });
+}
+
+dgk_status dgk_get_receivers(dgk_ctx* c, double* out, int capacity_rows, int* rows) {
+  if (!c || !rows) return fail(c, DGK_ERR_INVALID, "bad receiver arguments");
+  return guarded(c, [&] {
+    std::vector<double> h;
+    int r = fetch_receivers(*c, h);
+    *rows = r;
+    if (out) {
+      if (r > capacity_rows) throw InvalidArg("output buffer holds fewer rows than recorded");
+      std::copy(h.begin(), h.end(), out);
+    }
+  });
+}
+
 dgk_status dgk_rk4(dgk_ctx* c, int nsteps, double dt, const double* t_start) {
   if (!c) return fail(nullptr, DGK_ERR_INVALID, "NULL context");
   return guarded(c, [&] { rk4_impl(*c, nsteps, dt, t_start); });
@@ -649,6 +796,10 @@ struct dgk_batch {
     dgk::Ctx* ctx = nullptr;
     std::vector<int> sims;
     std::vector<int64_t> off;  // element offset of each member in the union
+    // receivers per member: element (member-local) and basis weights; columns in the group record
+    std::vector<std::vector<int64_t>> rel;
+    std::vector<std::vector<double>> rw;
+    int rcap = 0;
   };
   std::vector<Group> groups;
   std::vector<std::pair<int, int>> where;  // sim -> (group, member)
@@ -900,6 +1051,61 @@ dgk_status dgk_batch_get_state(dgk_batch* b, int sim, double* u) { return batch_
 dgk_status dgk_batch_rk4(dgk_batch* b, int nsteps, int schedule) {
   if (!b) return bfail(nullptr, DGK_ERR_INVALID, "NULL batch");
   return bguard(b, [&] { batch_rk4_impl(*b, nsteps, schedule); });
+}
+
+dgk_status dgk_batch_set_receivers(dgk_batch* b, int sim, int n, const double* xyz, const double* node_xyz,
+                                   int capacity) {
+  if (!b || sim < 0 || sim >= (int)b->where.size() || n < 0 || (n > 0 && (!xyz || !node_xyz)))
+    return bfail(b, DGK_ERR_INVALID, "bad receiver arguments");
+  return bguard(b, [&] {
+    DGK_HIP(hipStreamSynchronize(b->main));
+    auto w = b->where[sim];
+    auto& G = b->groups[w.first];
+    dgk::Ctx& c = *G.ctx;
+    G.rel.resize(G.sims.size());
+    G.rw.resize(G.sims.size());
+    dgk::RefElem R = dgk::make_ref_elem(c.dim, c.p);
+    std::vector<int64_t> el(n);
+    std::vector<double> wt, wi;
+    for (int i = 0; i < n; ++i) {
+      el[i] = dgk::locate_point(R, node_xyz, b->K[sim], xyz + 3 * i, wi);
+      wt.insert(wt.end(), wi.begin(), wi.end());
+    }
+    G.rel[w.second] = el;
+    G.rw[w.second] = wt;
+    G.rcap = std::max(G.rcap, capacity);
+    std::vector<int64_t> all;
+    std::vector<double> allw;
+    for (size_t k = 0; k < G.sims.size(); ++k) {
+      for (int64_t e : G.rel[k]) all.push_back(e + G.off[k]);
+      allw.insert(allw.end(), G.rw[k].begin(), G.rw[k].end());
+    }
+    dgk::install_receivers(c, all, allw, G.rcap);
+    if (b->graph) { DGK_HIP(hipGraphExecDestroy(b->graph)); b->graph = nullptr; }
+  });
+}
+
+dgk_status dgk_batch_get_receivers(dgk_batch* b, int sim, double* out, int capacity_rows, int* rows) {
+  if (!b || sim < 0 || sim >= (int)b->where.size() || !rows) return bfail(b, DGK_ERR_INVALID, "bad receiver arguments");
+  return bguard(b, [&] {
+    DGK_HIP(hipStreamSynchronize(b->main));
+    auto w = b->where[sim];
+    auto& G = b->groups[w.first];
+    std::vector<double> h;
+    const int r = dgk::fetch_receivers(*G.ctx, h);
+    size_t col0 = 0, n = 0;
+    if (w.second < (int)G.rel.size()) {
+      for (int k = 0; k < w.second; ++k) col0 += G.rel[k].size();
+      n = G.rel[w.second].size();
+    }
+    *rows = n ? r : 0;
+    if (out && n) {
+      if (r > capacity_rows) throw dgk::InvalidArg("output buffer holds fewer rows than recorded");
+      const size_t ng = G.ctx->rec.n;
+      for (int i = 0; i < r; ++i)
+        for (size_t j = 0; j < n; ++j) out[(size_t)i * n + j] = h[(size_t)i * ng + col0 + j];
+    }
+  });
 }
 
 dgk_status dgk_batch_prepare(dgk_batch* b, int nsteps, int schedule) {

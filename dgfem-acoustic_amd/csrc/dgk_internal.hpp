@@ -104,6 +104,15 @@ struct Ctx {
     int64_t n = 0;
   };
   std::vector<Src> sources;
+  // receivers: pressure interpolated at points (element Lagrange basis), recorded after every step
+  struct Rec {
+    int n = 0, cap = 0;
+    int* base = nullptr;     // [n] offset e*Np of the receiver's element in the pressure plane
+    void* w = nullptr;       // [n][Np] basis weights in the state type
+    void* out = nullptr;     // [cap][n] recorded pressure in the state type
+    int* counter = nullptr;  // rows recorded so far (device)
+  } rec;
+  void free_receivers();
   int64_t device_bytes = 0;
   // graph cache
   hipGraphExec_t graph = nullptr;
@@ -131,6 +140,7 @@ void scale_geometry(Ctx& c, int64_t e0, int64_t n, double s);
 // stage.hip
 void launch_stage(Ctx& c, StageArgs& a, hipStream_t s);
 void launch_source(Ctx& c, const Ctx::Src& s, double value, hipStream_t st);
+void launch_record(Ctx& c, hipStream_t st);  // no-op without receivers
 bool check_finite(Ctx& c);
 // FP64 -> state type; in the FP32 compensated form lo (if given) receives the bf16 low words
 void convert_state(Ctx& c, const double* src_dev, void* dst_dev, int64_t n, uint16_t* lo = nullptr);
@@ -143,6 +153,10 @@ extern thread_local std::string g_err;
 void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* o);
 void rk4_step(Ctx& c, double dt, hipStream_t s, hipEvent_t* ev = nullptr);
 double* staging(Ctx& c);
+struct RefElem;
+int64_t locate_point(const RefElem& R, const double* node_xyz, int64_t K, const double* x, std::vector<double>& w);
+void install_receivers(Ctx& c, const std::vector<int64_t>& elem, const std::vector<double>& w, int cap);
+int fetch_receivers(Ctx& c, std::vector<double>& out);
 
 
 }  // namespace dgk

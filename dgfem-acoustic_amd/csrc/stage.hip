@@ -572,6 +572,33 @@ __global__ void k_source(T* __restrict__ U, uint16_t* __restrict__ Ulo, const in
   }
 }
 
+// One workgroup: every thread interpolates receivers r = tid, tid+256, ... from the pressure plane
+// and writes row *counter of the record; the row index is read before and advanced after the
+// writes, so a captured graph records consecutive rows on every replay. Rows beyond cap are dropped.
+template <typename T>
+__global__ void k_record(const T* __restrict__ U, const uint16_t* __restrict__ Ulo, const int* __restrict__ base,
+                         const T* __restrict__ w, int n, int Np, T* __restrict__ out, int* __restrict__ counter,
+                         int cap) {
+  __shared__ int row;
+  if (threadIdx.x == 0) row = *counter;
+  __syncthreads();
+  const int r0 = row;
+  if (r0 < cap) {
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+      const int b = base[r];
+      T acc = 0;
+      for (int k = 0; k < Np; ++k) {
+        T u = U[b + k];
+        if (Ulo) u += (T)bf16_bits_to_f32(Ulo[b + k]);
+        acc += w[r * Np + k] * u;
+      }
+      out[(int64_t)r0 * n + r] = acc;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && r0 < cap) *counter = r0 + 1;
+}
+
 template <typename T>
 __global__ void k_finite(const T* __restrict__ x, int64_t n, int* __restrict__ flag) {
   int bad = 0;
@@ -677,6 +704,19 @@ void launch_source(Ctx& c, const Ctx::Src& src, double value, hipStream_t st) {
   else
     hipLaunchKernelGGL(k_source<float>, dim3(nb), dim3(256), 0, st, (float*)c.U, (uint16_t*)(c.comp ? c.Ulo : nullptr),
                        src.idx, src.n, (float)value);
+  DGK_HIP(hipGetLastError());
+}
+
+void launch_record(Ctx& c, hipStream_t st) {
+  if (!c.rec.n) return;
+  if (c.precision == DGK_FP64)
+    hipLaunchKernelGGL(k_record<double>, dim3(1), dim3(256), 0, st, (const double*)c.U, (const uint16_t*)nullptr,
+                       c.rec.base, (const double*)c.rec.w, c.rec.n, c.Np, (double*)c.rec.out, c.rec.counter,
+                       c.rec.cap);
+  else
+    hipLaunchKernelGGL(k_record<float>, dim3(1), dim3(256), 0, st, (const float*)c.U,
+                       (const uint16_t*)(c.comp ? c.Ulo : nullptr), c.rec.base, (const float*)c.rec.w, c.rec.n,
+                       c.Np, (float*)c.rec.out, c.rec.counter, c.rec.cap);
   DGK_HIP(hipGetLastError());
 }
 

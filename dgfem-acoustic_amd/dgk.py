@@ -189,7 +189,8 @@ DEV_SYMBOLS = ("dgk_create", "dgk_destroy", "dgk_last_error", "dgk_info_get", "d
                "dgk_set_sources", "dgk_rk4", "dgk_euler", "dgk_rhs", "dgk_sync", "dgk_check_finite",
                "dgk_connectivity", "dgk_rk4_timed", "dgk_variant_name", "dgk_batch_create", "dgk_batch_destroy",
                "dgk_batch_last_error", "dgk_batch_set_state", "dgk_batch_get_state", "dgk_batch_rk4",
-               "dgk_batch_sync", "dgk_batch_groups", "dgk_batch_prepare", "dgk_batch_rk4_timed")
+               "dgk_batch_sync", "dgk_batch_groups", "dgk_batch_prepare", "dgk_batch_rk4_timed",
+               "dgk_set_receivers", "dgk_get_receivers", "dgk_batch_set_receivers", "dgk_batch_get_receivers")
 
 
 def device():
@@ -225,6 +226,11 @@ def device():
         L.dgk_batch_get_state.argtypes = [vp, ctypes.c_int, P(ctypes.c_double)]
         L.dgk_batch_rk4.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.dgk_batch_sync.argtypes = [vp]
+        L.dgk_set_receivers.argtypes = [vp, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double), ctypes.c_int]
+        L.dgk_get_receivers.argtypes = [vp, P(ctypes.c_double), ctypes.c_int, P(ctypes.c_int)]
+        L.dgk_batch_set_receivers.argtypes = [vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double),
+                                              ctypes.c_int]
+        L.dgk_batch_get_receivers.argtypes = [vp, ctypes.c_int, P(ctypes.c_double), ctypes.c_int, P(ctypes.c_int)]
         L.dgk_batch_prepare.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.dgk_batch_rk4_timed.argtypes = [vp, ctypes.c_int, P(ctypes.c_double)]
         L.dgk_batch_groups.argtypes = [vp, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int64),
@@ -314,6 +320,21 @@ class Solver:
         self._check(device().dgk_connectivity(self.h, *[_p(a, ctypes.c_int32) for a in arrs]))
         return dict(zip(("nbr", "nface", "perm", "sign"), (a.reshape(self.K, -1) for a in arrs)))
 
+    def set_receivers(self, xyz, capacity):
+        """Record the pressure at points xyz (n x 3) after every step, up to `capacity` steps."""
+        x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        self._nrec = len(x)
+        self._check(device().dgk_set_receivers(self.h, len(x), _p(x), _p(self._xyz), int(capacity)))
+
+    def get_receivers(self):
+        """Recorded pressure, shape (steps, n)."""
+        L = device()
+        rows = ctypes.c_int()
+        self._check(L.dgk_get_receivers(self.h, None, 0, ctypes.byref(rows)))
+        out = np.zeros((rows.value, getattr(self, "_nrec", 0)))
+        self._check(L.dgk_get_receivers(self.h, _p(out), rows.value, ctypes.byref(rows)))
+        return out
+
     def rk4_timed(self, nsteps, dt):
         s, t = ctypes.c_double(), ctypes.c_double()
         self._check(device().dgk_rk4_timed(self.h, int(nsteps), float(dt), ctypes.byref(s), ctypes.byref(t)))
@@ -385,6 +406,25 @@ class Batch:
 
     def sync(self):
         self._check(device().dgk_batch_sync(self.h))
+
+    def set_receivers(self, i, xyz, capacity):
+        """Record member i's pressure at points xyz (n x 3) after every step (group capacity = max asked)."""
+        m = self.meshes[i]
+        x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        nx = np.ascontiguousarray(m.node_xyz, dtype=np.float64)
+        if not hasattr(self, "_nrec"):
+            self._nrec = {}
+        self._nrec[i] = len(x)
+        self._check(device().dgk_batch_set_receivers(self.h, int(i), len(x), _p(x), _p(nx), int(capacity)))
+
+    def get_receivers(self, i):
+        """Member i's recorded pressure, shape (steps, n)."""
+        L = device()
+        rows = ctypes.c_int()
+        self._check(L.dgk_batch_get_receivers(self.h, int(i), None, 0, ctypes.byref(rows)))
+        out = np.zeros((rows.value, getattr(self, "_nrec", {}).get(i, 0)))
+        self._check(L.dgk_batch_get_receivers(self.h, int(i), _p(out), rows.value, ctypes.byref(rows)))
+        return out
 
     def prepare(self, nsteps, schedule=SCHEDULE_PARALLEL):
         """Capture the graph for rk4(nsteps, schedule) ahead of time."""

@@ -115,6 +115,14 @@ dgk_status dgk_check_finite(dgk_ctx* ctx);
  *   perm[e*Nf+j]  index of the vertex permutation taking this face's vertex order to the
  *                 neighbour's (-1 on the boundary)
  *   sign[e*Nf+j]  interior penalty sign (+1/-1; 0 on the boundary) */
+/* Receivers: pressure at n points (xyz[3n]), interpolated with the Lagrange basis of the element
+ * in which the point is most interior (DG fields jump across faces; on a face either side may be
+ * taken) and recorded after every time step (rk4 and euler), up to `capacity` steps. node_xyz is the mesh's
+ * element-node coordinate array (as in dgk_set_sources). Setting receivers clears the record.
+ * No reference counterpart in the solver (the reference writes whole-field views only). */
+dgk_status dgk_set_receivers(dgk_ctx* ctx, int n, const double* xyz, const double* node_xyz, int capacity);
+/* rows = steps recorded; out (may be NULL to query rows) receives rows x n values, row-major */
+dgk_status dgk_get_receivers(dgk_ctx* ctx, double* out, int capacity_rows, int* rows);
 dgk_status dgk_connectivity(dgk_ctx* ctx, int32_t* nbr, int32_t* nface, int32_t* perm, int32_t* sign);
 
 /* Timed run for measurement: nsteps RK4 steps with a HIP event pair around every stage
@@ -140,6 +148,11 @@ dgk_status dgk_batch_set_state(dgk_batch* b, int sim, const double* u);
 dgk_status dgk_batch_get_state(dgk_batch* b, int sim, double* u);
 /* schedule: 0 = groups as parallel graph branches, 1 = groups in sequence on one stream */
 dgk_status dgk_batch_rk4(dgk_batch* b, int nsteps, int schedule);
+/* receivers of member sim (node_xyz: that member's mesh); members of one group share a record
+ * capacity (the largest requested) and setting any member's receivers clears the group's record */
+dgk_status dgk_batch_set_receivers(dgk_batch* b, int sim, int n, const double* xyz, const double* node_xyz,
+                                   int capacity);
+dgk_status dgk_batch_get_receivers(dgk_batch* b, int sim, double* out, int capacity_rows, int* rows);
 /* capture (and keep) the graph that dgk_batch_rk4(b, nsteps, schedule) will replay, without running it */
 dgk_status dgk_batch_prepare(dgk_batch* b, int nsteps, int schedule);
 /* nsteps RK4 steps group by group without a graph, with a HIP event pair around every stage

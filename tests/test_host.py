@@ -174,3 +174,34 @@ def test_cli_wrong_argc_returns_e2big():
         pytest.skip("dgalerkin not built")
     r = subprocess.run([exe], capture_output=True)
     assert r.returncode == 7  # E2BIG
+
+
+@pytest.mark.parametrize("dim,p", [(1, 3), (2, 2), (2, 5), (3, 3), (3, 5)])
+def test_interpolation_helper_reproduces_polynomials(dim, p):
+    """The receiver tests' independent interpolation is exact for degree-p polynomials."""
+    from interp import sample
+    from workloads import conf
+
+    bcs = conf("Wall=Reflecting\n")
+    if dim == 1:
+        m = dgk.gen_line(5, 1.0, p, [1, 1], "1:Wall", bcs)
+    elif dim == 2:
+        m = dgk.gen_rect(3, 2, 1.0, 0.8, p, [1, 1, 1, 1], "1:Wall", bcs)
+    else:
+        m = dgk.gen_box(2, 2, 1, 1.0, 0.8, 0.7, p, 0.1, 3, [1] * 6, "1:Wall", bcs)
+    rng = np.random.default_rng(1)
+    coef = rng.standard_normal(4)
+
+    def f(x):
+        s = coef[0] + coef[1] * x[..., 0]
+        if dim > 1:
+            s = s + coef[2] * x[..., 1] ** 2 * x[..., 0] ** (p - 2 if p > 2 else 0)
+        if dim > 2:
+            s = s + coef[3] * x[..., 2] ** p
+        return s + x[..., 0] ** p
+
+    u = np.zeros((4, m.K * m.Np))
+    u[0] = f(m.node_xyz)
+    ext = np.array([1.0, 0.8 if dim > 1 else 0.0, 0.7 if dim > 2 else 0.0])
+    pts = 0.05 * ext + 0.9 * ext * rng.random((6, 3))
+    assert np.allclose(sample(m, u, pts), f(pts), rtol=1e-10, atol=1e-10)
