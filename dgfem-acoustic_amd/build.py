@@ -1,7 +1,7 @@
 """Build recipe for the dgalerkin MI355X libraries (run by __graft_entry__.build()).
 
 Products, all in this directory:
-  libdgk.so      HIP kernels + C ABI (hipcc, gfx950)
+  libdgk.so      HIP kernels + C ABI (hipcc, gfx950): DG solver (dgk_*) and surrogate (dgs_*)
   libdgkhost.so  MSH / config / mesh generators (g++)
   dgalerkin      the command-line driver (links both)
 """
@@ -16,7 +16,7 @@ ARCH = "gfx950"
 # Packed FP32 instructions are disabled in all device code.
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-DEV_SOURCES = ["csrc/dgk_api.cpp", "csrc/setup.hip", "csrc/stage.hip"]
+DEV_SOURCES = ["csrc/dgk_api.cpp", "csrc/setup.hip", "csrc/stage.hip", "csrc/surrogate.hip"]
 HOST_SOURCES = ["host/host_capi.cpp", "host/meshio.cpp", "host/config.cpp", "host/meshgen.cpp"]
 
 
@@ -31,7 +31,28 @@ def _run(cmd):
 
 
 def build_device():
-    _run([HIPCC] + HIP_FLAGS + ["-shared", "-o", "libdgk.so"] + DEV_SOURCES)
+    """Compile the device sources in parallel, then link libdgk.so."""
+    objs, procs = [], []
+    os.makedirs(os.path.join(HERE, "_obj"), exist_ok=True)
+    for src in DEV_SOURCES:
+        obj = os.path.join("_obj", os.path.basename(src) + ".o")
+        cmd = [HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj]
+        print("+", " ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd, cwd=HERE, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                            text=True)))
+        objs.append(obj)
+    failed = []
+    for cmd, p in procs:
+        out, _ = p.communicate()
+        out = "\n".join(l for l in out.splitlines()
+                        if "packed-fp32-ops" not in l and "is not a recognized feature" not in l)
+        if out.strip():
+            print(out, flush=True)
+        if p.returncode:
+            failed.append(" ".join(cmd))
+    if failed:
+        raise RuntimeError("build step failed: " + "; ".join(failed))
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", "libdgk.so"] + objs)
 
 
 def build_host():

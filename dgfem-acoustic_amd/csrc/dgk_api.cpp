@@ -379,71 +379,125 @@ void rk4_step(Ctx& c, double dt, hipStream_t s, hipEvent_t* ev) {
   launch_record(c, s);
 }
 
-// Locate point x in a mesh given by its element node coordinates (Gmsh order, vertices first):
-// the element whose smallest barycentric coordinate of x is largest. Returns the element and the
-// element's Lagrange basis at x (the pressure at x is sum_n w[n] p[e*Np + n]).
-int64_t locate_point(const RefElem& R, const double* node_xyz, int64_t K, const double* x, std::vector<double>& w) {
+namespace {
+
+// Barycentric coordinates of x in element e (normal equations of x - v0 = sum_a l_a (v_a - v0));
+// returns the smallest coordinate (>= 0 inside).
+double bary_score(int d, int Np, const double* node_xyz, int64_t e, const double* x, double* l) {
+  const double* X = node_xyz + (size_t)e * Np * 3;
+  double J[3][3] = {}, G[3][3] = {}, r[3] = {}, b[3] = {};
+  for (int a = 0; a < d; ++a)
+    for (int c = 0; c < 3; ++c) J[c][a] = X[3 * (a + 1) + c] - X[c];
+  for (int c = 0; c < 3; ++c) r[c] = x[c] - X[c];
+  for (int a = 0; a < d; ++a) {
+    for (int bb = 0; bb < d; ++bb)
+      for (int c = 0; c < 3; ++c) G[a][bb] += J[c][a] * J[c][bb];
+    for (int c = 0; c < 3; ++c) b[a] += J[c][a] * r[c];
+  }
+  std::vector<ld> Gi(d * d);
+  for (int a = 0; a < d; ++a)
+    for (int bb = 0; bb < d; ++bb) Gi[a * d + bb] = G[a][bb];
+  invert(Gi, d);
+  double sum = 0;
+  for (int a = 0; a < d; ++a) {
+    ld t = 0;
+    for (int bb = 0; bb < d; ++bb) t += Gi[a * d + bb] * b[bb];
+    l[a + 1] = (double)t;
+    sum += l[a + 1];
+  }
+  l[0] = 1 - sum;
+  double score = l[0];
+  for (int a = 1; a <= d; ++a) score = std::min(score, l[a]);
+  return score;
+}
+
+}  // namespace
+
+// Locate points xyz[n][3] in a mesh given by its element node coordinates (Gmsh order, vertices
+// first): for each point the element whose smallest barycentric coordinate of the point is
+// largest. Elements are binned by bounding box into a uniform grid of about K cells, so each
+// point tests only the elements of its cell. Returns the elements; w receives, per point, the
+// element's Lagrange basis at the point (pressure = sum_n w[n] p[e*Np + n]).
+std::vector<int64_t> locate_points(const RefElem& R, const double* node_xyz, int64_t K, int n, const double* xyz,
+                                   std::vector<double>& w) {
   const int d = R.dim, Np = R.Np;
-  int64_t best = -1;
-  double best_score = -1e300, best_l[4] = {0, 0, 0, 0};
+  std::vector<int64_t> out(n, -1);
+  w.assign((size_t)n * Np, 0.0);
+  if (!n) return out;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int64_t e = 0; e < K; ++e)
+    for (int v = 0; v <= d; ++v)
+      for (int c = 0; c < 3; ++c) {
+        const double q = node_xyz[((size_t)e * Np + v) * 3 + c];
+        lo[c] = std::min(lo[c], q);
+        hi[c] = std::max(hi[c], q);
+      }
+  int g[3] = {1, 1, 1};
+  const int gd = (int)std::max(1.0, std::floor(std::pow((double)K, 1.0 / d)));
+  for (int c = 0; c < d; ++c) g[c] = gd;
+  auto cell = [&](int c, double q) {
+    const double span = hi[c] - lo[c];
+    int i = span > 0 ? (int)std::floor((q - lo[c]) / span * g[c]) : 0;
+    return std::min(std::max(i, 0), g[c] - 1);
+  };
+  // bin elements (CSR over cells)
+  const int64_t ncell = (int64_t)g[0] * g[1] * g[2];
+  std::vector<int64_t> cnt(ncell + 1, 0);
+  std::vector<std::array<int, 6>> box(K);
   for (int64_t e = 0; e < K; ++e) {
-    const double* X = node_xyz + (size_t)e * Np * 3;
-    // bounding box prefilter with a tolerance of the box size
-    double lo[3], hi[3];
+    double a[3], b[3];
     for (int c = 0; c < 3; ++c) {
-      lo[c] = hi[c] = X[c];
+      a[c] = b[c] = node_xyz[((size_t)e * Np) * 3 + c];
       for (int v = 1; v <= d; ++v) {
-        lo[c] = std::min(lo[c], X[3 * v + c]);
-        hi[c] = std::max(hi[c], X[3 * v + c]);
+        a[c] = std::min(a[c], node_xyz[((size_t)e * Np + v) * 3 + c]);
+        b[c] = std::max(b[c], node_xyz[((size_t)e * Np + v) * 3 + c]);
       }
     }
-    bool out = false;
     for (int c = 0; c < 3; ++c) {
-      const double tol = 1e-9 + 1e-6 * (hi[c] - lo[c]);
-      if (x[c] < lo[c] - tol || x[c] > hi[c] + tol) out = true;
+      const double tol = 1e-9 * (1 + hi[c] - lo[c]);
+      box[e][c] = cell(c, a[c] - tol);
+      box[e][3 + c] = cell(c, b[c] + tol);
     }
-    if (out && best >= 0) continue;
-    // barycentric coordinates by the normal equations of x - v0 = sum_a l_a (v_a - v0)
-    double J[3][3] = {}, G[3][3] = {}, r[3] = {}, b[3] = {};
-    for (int a = 0; a < d; ++a)
-      for (int c = 0; c < 3; ++c) J[c][a] = X[3 * (a + 1) + c] - X[c];
-    for (int c = 0; c < 3; ++c) r[c] = x[c] - X[c];
-    for (int a = 0; a < d; ++a) {
-      for (int bb = 0; bb < d; ++bb)
-        for (int c = 0; c < 3; ++c) G[a][bb] += J[c][a] * J[c][bb];
-      for (int c = 0; c < 3; ++c) b[a] += J[c][a] * r[c];
-    }
-    std::vector<ld> Gi(d * d);
-    for (int a = 0; a < d; ++a)
-      for (int bb = 0; bb < d; ++bb) Gi[a * d + bb] = G[a][bb];
-    invert(Gi, d);
-    double l[4];
-    double sum = 0;
-    for (int a = 0; a < d; ++a) {
-      ld t = 0;
-      for (int bb = 0; bb < d; ++bb) t += Gi[a * d + bb] * b[bb];
-      l[a + 1] = (double)t;
-      sum += l[a + 1];
-    }
-    l[0] = 1 - sum;
-    double score = l[0];
-    for (int a = 1; a <= d; ++a) score = std::min(score, l[a]);
-    if (score > best_score) {
-      best_score = score;
-      best = e;
-      for (int a = 0; a <= d; ++a) best_l[a] = l[a];
-    }
+    for (int i = box[e][0]; i <= box[e][3]; ++i)
+      for (int j = box[e][1]; j <= box[e][4]; ++j)
+        for (int k = box[e][2]; k <= box[e][5]; ++k) ++cnt[((int64_t)i * g[1] + j) * g[2] + k + 1];
   }
-  if (best < 0 || best_score < -1e-6) throw InvalidArg("receiver point outside the mesh");
-  std::array<ld, 3> ref{0, 0, 0};
-  if (d == 1) ref[0] = best_l[1] - best_l[0];
-  else
-    for (int a = 0; a < d; ++a) ref[a] = best_l[a + 1];
-  std::vector<ld> phi(Np);
-  R.eval_basis(ref, phi.data(), nullptr);
-  w.assign(Np, 0);
-  for (int n = 0; n < Np; ++n) w[n] = (double)phi[n];
-  return best;
+  for (int64_t c = 0; c < ncell; ++c) cnt[c + 1] += cnt[c];
+  std::vector<int64_t> items(cnt[ncell]), fill(cnt.begin(), cnt.end() - 1);
+  for (int64_t e = 0; e < K; ++e)
+    for (int i = box[e][0]; i <= box[e][3]; ++i)
+      for (int j = box[e][1]; j <= box[e][4]; ++j)
+        for (int k = box[e][2]; k <= box[e][5]; ++k) items[fill[((int64_t)i * g[1] + j) * g[2] + k]++] = e;
+  for (int p = 0; p < n; ++p) {
+    const double* x = xyz + 3 * p;
+    int64_t best = -1;
+    double best_score = -1e300, best_l[4] = {0, 0, 0, 0}, l[4];
+    bool in_box = true;
+    for (int c = 0; c < d; ++c)
+      if (x[c] < lo[c] - 1e-9 * (1 + hi[c] - lo[c]) || x[c] > hi[c] + 1e-9 * (1 + hi[c] - lo[c])) in_box = false;
+    if (in_box) {
+      const int64_t ci = ((int64_t)cell(0, x[0]) * g[1] + cell(1, x[1])) * g[2] + cell(2, x[2]);
+      for (int64_t it = cnt[ci]; it < cnt[ci + 1]; ++it) {
+        const double sc = bary_score(d, Np, node_xyz, items[it], x, l);
+        if (sc > best_score || (sc == best_score && items[it] < best)) {
+          best_score = sc;
+          best = items[it];
+          std::copy(l, l + 4, best_l);
+        }
+      }
+    }
+    if (best < 0 || best_score < -1e-6)
+      throw InvalidArg("receiver point " + std::to_string(p) + " lies outside the mesh");
+    std::array<ld, 3> ref{0, 0, 0};
+    if (d == 1) ref[0] = best_l[1] - best_l[0];
+    else
+      for (int a = 0; a < d; ++a) ref[a] = best_l[a + 1];
+    std::vector<ld> phi(Np);
+    R.eval_basis(ref, phi.data(), nullptr);
+    for (int k = 0; k < Np; ++k) w[(size_t)p * Np + k] = (double)phi[k];
+    out[p] = best;
+  }
+  return out;
 }
 
 // Install receivers (element, weights) on the context, with room for cap recorded steps.
@@ -680,12 +734,8 @@ dgk_status dgk_set_receivers(dgk_ctx* c, int n, const double* xyz, const double*
   if (!c || n < 0 || (n > 0 && (!xyz || !node_xyz))) return fail(c, DGK_ERR_INVALID, "bad receiver arguments");
   return guarded(c, [&] {
     RefElem R = make_ref_elem(c->dim, c->p);
-    std::vector<int64_t> elem(n);
-    std::vector<double> w, wi;
-    for (int i = 0; i < n; ++i) {
-      elem[i] = locate_point(R, node_xyz, c->K, xyz + 3 * i, wi);
-      w.insert(w.end(), wi.begin(), wi.end());
-    }
+    std::vector<double> w;
+    std::vector<int64_t> elem = locate_points(R, node_xyz, c->K, n, xyz, w);
     install_receivers(*c, elem, w, capacity);
   });
 }
@@ -1065,12 +1115,8 @@ dgk_status dgk_batch_set_receivers(dgk_batch* b, int sim, int n, const double* x
     G.rel.resize(G.sims.size());
     G.rw.resize(G.sims.size());
     dgk::RefElem R = dgk::make_ref_elem(c.dim, c.p);
-    std::vector<int64_t> el(n);
-    std::vector<double> wt, wi;
-    for (int i = 0; i < n; ++i) {
-      el[i] = dgk::locate_point(R, node_xyz, b->K[sim], xyz + 3 * i, wi);
-      wt.insert(wt.end(), wi.begin(), wi.end());
-    }
+    std::vector<double> wt;
+    std::vector<int64_t> el = dgk::locate_points(R, node_xyz, b->K[sim], n, xyz, wt);
     G.rel[w.second] = el;
     G.rw[w.second] = wt;
     G.rcap = std::max(G.rcap, capacity);

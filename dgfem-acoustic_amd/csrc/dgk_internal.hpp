@@ -154,7 +154,8 @@ void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* 
 void rk4_step(Ctx& c, double dt, hipStream_t s, hipEvent_t* ev = nullptr);
 double* staging(Ctx& c);
 struct RefElem;
-int64_t locate_point(const RefElem& R, const double* node_xyz, int64_t K, const double* x, std::vector<double>& w);
+std::vector<int64_t> locate_points(const RefElem& R, const double* node_xyz, int64_t K, int n, const double* xyz,
+                                   std::vector<double>& w);
 void install_receivers(Ctx& c, const std::vector<int64_t>& elem, const std::vector<double>& w, int cap);
 int fetch_receivers(Ctx& c, std::vector<double>& out);
 

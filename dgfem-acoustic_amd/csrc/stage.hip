@@ -43,7 +43,7 @@ constexpr int kBlock = 256;
 template <typename T, int D, int P>
 __global__ __launch_bounds__(kBlock) void k_stage_lds(StageArgs a) {
   using S = Sz<D, P>;
-  constexpr int NP = S::NP, NF = S::NF, NFP = S::NFP, NPERM = S::NPERM;
+  constexpr int NP = S::NP, NF = S::NF, NFP = S::NFP;
   constexpr int TE = kBlock / NP;
   constexpr int NFF = NF * NFP;
   __shared__ T sU[4][TE * NP];
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_lds(StageArgs a) {
   const T* __restrict__ Jinv = (const T*)a.Jinv;
   const T* __restrict__ W = (const T*)a.W;
   const T* __restrict__ LIFT = (const T*)a.LIFT;
-  const T dt = (T)a.dt, c0 = (T)a.c0, rho = (T)a.rho0;
+  const T dt = (T)a.dt, c0 = (T)a.c0;
   const T rc2 = (T)(a.rho0 * a.c0 * a.c0), irho = (T)(1.0 / a.rho0);
   const T v0[3] = {(T)a.v0[0], (T)a.v0[1], (T)a.v0[2]};
   const int stage = a.stage;

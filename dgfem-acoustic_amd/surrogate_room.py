@@ -1,0 +1,136 @@
+"""Learned receiver surrogate for the shoebox room at order 3, end to end on one GPU.
+
+Room: 20 x 15 x 8 m, a 2 x 2 x 1 m rigid block on the floor at a seeded grid-aligned spot
+(seed 4), Kuhn tets of 0.25 m (920,064 elements), order 3, walls/floor/block reflecting, ceiling
+absorbing, c0 = 343. Impulse responses: Gaussian pulses of the 500 Hz-equivalent width
+sigma = c0/(2 pi 500) released at seeded source points; pressure recorded at seeded receiver
+points after every one of 2048 RK4 steps (FP32 batched DG solver, one batch member per source).
+Data: 36 sources x 128 receivers = 4608 pairs (seed 1000), 4096 for training, 512 held out.
+Network: 6 x 512 (5 SiLU layers of 512 + a 2048-sample output layer), 128 Fourier frequencies,
+BF16 MFMA kernels with FP32 accumulation; two networks (seeds 1, 2) whose disagreement is the
+error estimate; queries above the 2% estimate fall back to a DG solve.
+
+Prints one JSON line with the held-out error distribution, the gate's acceptance, and the query
+time against the DG solve time.
+
+Usage: python dgfem-acoustic_amd/surrogate_room.py [--sources 36] [--receivers 128] [--steps 4000]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "tests"))
+
+import dgk  # noqa: E402
+import surrogate  # noqa: E402
+import workloads as wl  # noqa: E402
+
+
+def log(msg):
+    print(f"[surrogate_room] {msg}", file=sys.stderr, flush=True)
+
+
+def points(rng, n, sigma, block_box, L):
+    """Uniform points at least 3 sigma from the walls and outside the block (grown by 3 sigma)."""
+    lo, hi = np.full(3, 3 * sigma), np.asarray(L) - 3 * sigma
+    out = []
+    while len(out) < n:
+        p = lo + (hi - lo) * rng.random(3)
+        if np.all(p > block_box[0] - 3 * sigma) and np.all(p < block_box[1] + 3 * sigma):
+            continue
+        out.append(p)
+    return np.array(out)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--sources", type=int, default=36)
+    ap.add_argument("--receivers", type=int, default=128)
+    ap.add_argument("--nsteps", type=int, default=2048)
+    ap.add_argument("--steps", type=int, default=4000, help="training steps per network")
+    ap.add_argument("--h", type=float, default=0.25)
+    ap.add_argument("--freq-sigma", type=float, default=0.1)
+    ap.add_argument("--out", default="")
+    args = ap.parse_args()
+    L = (20.0, 15.0, 8.0)
+    h = args.h
+    nx, ny = int(round(L[0] / h)), int(round(L[1] / h))
+    brng = np.random.Generator(np.random.MT19937(4))
+    bx, by = int(2 / h), int(2 / h)
+    i0, j0 = int(brng.integers(0, nx - bx)), int(brng.integers(0, ny - by))
+    block = ((i0, j0, 0), (i0 + bx, j0 + by, int(round(1 / h))))
+    block_box = (np.array([i0 * h, j0 * h, 0.0]), np.array([(i0 + bx) * h, (j0 + by) * h, 1.0]))
+    t0 = time.perf_counter()
+    m, ph, dt, sigma = wl.room_box(*L, h, 3, block=block)
+    m.release()
+    log(f"room K={m.K} Np={m.Np} dt={dt:.3e} sigma={sigma:.4f} ({time.perf_counter() - t0:.1f} s)")
+    rng = np.random.default_rng(1000)
+    S, R = args.sources, args.receivers
+    src = points(rng, S, sigma, block_box, L)
+    rcv = np.stack([points(rng, R, sigma, block_box, L) for _ in range(S)])
+    t0 = time.perf_counter()
+    series = surrogate.dg_impulse_responses(m, ph, dt, src, rcv, args.nsteps, sigma, 1, per_batch=12)
+    t_data = time.perf_counter() - t0
+    log(f"DG data: {S} sources x {R} receivers x {args.nsteps} samples in {t_data:.1f} s")
+    # one plain DG solve (single context, one source, all its receivers) for the per-query cost
+    s = dgk.Solver(m, ph["c0"], ph["rho0"], precision=dgk.FP32, flux_sign=dgk.SIGN_STABLE)
+    u = np.zeros((4, m.K * m.Np))
+    u[0] = np.exp(-((m.node_xyz - src[0]) ** 2).sum(axis=1) / (2 * sigma ** 2))
+    s.set_state(u)
+    s.set_receivers(rcv[0], args.nsteps)
+    s.sync()
+    t0 = time.perf_counter()
+    s.rk4(args.nsteps, dt)
+    s.sync()
+    t_dg = time.perf_counter() - t0
+    single = s.get_receivers().T
+    s.close()
+    consistency = float(np.linalg.norm(single - series[0]) / np.linalg.norm(single))
+    X_src, X_rcv, T = np.repeat(src, R, axis=0), rcv.reshape(-1, 3), series.reshape(S * R, -1)
+    perm = rng.permutation(S * R)
+    tr, te = perm[:4096], perm[4096:4096 + 512]
+    sur = surrogate.Surrogate(np.zeros(3), np.array(L), out_dim=args.nsteps, scale=float(np.abs(T).max()),
+                              freq_sigma=args.freq_sigma)
+    t0 = time.perf_counter()
+    losses = sur.fit(X_src[tr], X_rcv[tr], T[tr], args.steps, batch=256, lr=1e-3)
+    t_train = time.perf_counter() - t0
+    # relative errors are stated over held-out pairs the pulse reaches within the window (series
+    # norm above 1e-3 of the largest); the rest are silent and counted separately
+    audible = np.linalg.norm(T[te], axis=1) > 1e-3 * np.linalg.norm(T, axis=1).max()
+    te = te[audible]
+    y, est = sur.predict(X_src[te], X_rcv[te])
+    err = surrogate.rel_l2(y, T[te])
+    ytr, _ = sur.predict(X_src[tr[:512]], X_rcv[tr[:512]])
+    err_tr = surrogate.rel_l2(ytr, T[tr[:512]])
+    t_q = sur.nets[0].forward_ms(256, 20) * len(sur.nets) / 256 * 1e-3
+    acc = est <= 0.02
+    out = {
+        "room": {"elements": m.K, "order": 3, "dofs": 4 * m.K * m.Np, "dt": dt, "samples": args.nsteps},
+        "data": {"sources": S, "receivers_per_source": R, "pairs_train": len(tr), "pairs_held_out": 512,
+                 "held_out_audible": int(audible.sum()),
+                 "dg_seconds": t_data, "batched_vs_single_rel_l2": consistency},
+        "network": {"params": sur.nets[0].nparam, "freq_sigma": args.freq_sigma, "train_steps": args.steps,
+                    "train_seconds": t_train, "final_loss": [float(l[-50:].mean()) for l in losses]},
+        "held_out_rel_l2": {"median": float(np.median(err)), "mean": float(err.mean()),
+                            "p10": float(np.percentile(err, 10)), "frac_below_2pct": float((err <= 0.02).mean())},
+        "train_rel_l2_median": float(np.median(err_tr)),
+        "gate": {"tol": 0.02, "accepted": float(acc.mean()),
+                 "accepted_err_median": float(np.median(err[acc])) if acc.any() else None,
+                 "corr_estimate_error": float(np.corrcoef(est, err)[0, 1])},
+        "timing": {"query_us": t_q * 1e6, "dg_solve_s": t_dg, "speedup": t_dg / t_q},
+    }
+    line = json.dumps(out)
+    print(line)
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(line + "\n")
+
+
+if __name__ == "__main__":
+    main()

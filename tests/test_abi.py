@@ -18,7 +18,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(dg[kh]_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(dg[khs]_[a-z0-9_]+)\s*\(", text)))
 
 
 def ensure_device_lib():
@@ -34,6 +34,25 @@ def test_libdgk_exports_header():
     for n in names:
         assert hasattr(lib, n), n
     assert set(names) == set(dgk.DEV_SYMBOLS)
+
+
+def test_libdgk_exports_surrogate_header():
+    import surrogate
+
+    ensure_device_lib()
+    lib = ctypes.CDLL(dgk.DEV_LIB)
+    names = declared("dgk_surrogate.h")
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(surrogate.SYMBOLS)
+
+
+def test_surrogate_rejects_bad_shapes_before_device_use():
+    import surrogate
+
+    ensure_device_lib()
+    with pytest.raises(RuntimeError, match="multiples of 64"):
+        surrogate.Net(width=100)
 
 
 def test_libdgkhost_exports_header():

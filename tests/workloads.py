@@ -125,3 +125,14 @@ def ensemble_state(mesh, pulse):
 def dof_count(meshes):
     """Counted unknowns: (dim+1) fields per node."""
     return sum((m.dim + 1) * m.K * m.Np for m in meshes)
+
+
+def room_box(Lx, Ly, Lz, h, order, block=None, seed=4):
+    """Shoebox room of Kuhn tets of size h: walls and floor reflecting, ceiling absorbing, c0 = 343;
+    optional block obstacle ((i0,j0,k0), (i1,j1,k1)) in cells. Returns (mesh, phys, dt, sigma) with
+    the 500 Hz-equivalent pulse width sigma = c0 / (2 pi 500)."""
+    nx, ny, nz = int(round(Lx / h)), int(round(Ly / h)), int(round(Lz / h))
+    m = dgk.gen_box(nx, ny, nz, Lx, Ly, Lz, order, 0.0, 0, [1, 1, 1, 1, 1, 2], "1:Walls;2:Ceiling",
+                    conf("Walls=Reflecting\nCeiling=Absorbing\nc0=343\n"), block=block, block_tag=1)
+    phys = dict(c0=343.0, rho0=1.225, v0=(0.0, 0.0, 0.0))
+    return m, phys, cfl_dt(h, 343.0, order), 343.0 / (2 * np.pi * 500.0)
