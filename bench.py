@@ -113,18 +113,32 @@ def batch_leg(steps, warmup, members_checked=8, n=256):
     groups = b.groups()
     log(f"batch: device setup and state upload {time.perf_counter() - t0:.1f} s, {len(groups)} groups")
     b.rk4(warmup)
+    # schedule: groups as parallel graph branches or in sequence, whichever runs a short
+    # captured trial faster on this GPU
+    trial = {}
+    for sched in (dgk.SCHEDULE_PARALLEL, dgk.SCHEDULE_SEQUENTIAL):
+        b.prepare(10, sched)
+        b.sync()
+        t0 = time.perf_counter()
+        b.rk4(10, sched)
+        b.sync()
+        trial[sched] = time.perf_counter() - t0
+    # parallel branches make the per-kernel trace durations overlap; take them only when clearly faster
+    schedule = dgk.SCHEDULE_PARALLEL if trial[0] < 0.97 * trial[1] else dgk.SCHEDULE_SEQUENTIAL
+    log(f"batch: schedule trial (10 steps) parallel {trial[0]:.3f} s, sequential {trial[1]:.3f} s")
+    total_before = warmup + 20
     t0 = time.perf_counter()
-    b.prepare(steps)
+    b.prepare(steps, schedule)
     log(f"batch: captured {steps} steps x {len(groups)} groups in {time.perf_counter() - t0:.1f} s")
     b.sync()
     t0 = time.perf_counter()
-    b.rk4(steps)
+    b.rk4(steps, schedule)
     b.sync()
     wall = time.perf_counter() - t0
     dof = wl.dof_count(meshes)
     nk = 10
     gms = b.rk4_timed(nk)
-    total = warmup + steps + nk
+    total = total_before + steps + nk
     gtab = []
     for g, ms in zip(groups, gms):
         Np = element_sizes(g["dim"], g["order"])[0]
@@ -162,6 +176,7 @@ def batch_leg(steps, warmup, members_checked=8, n=256):
         del x["stage_s"], x["bytes"]
     return dict(rate=dof * 4 * steps / wall, wall=wall, dof=dof, elements=sum(m.K for m in meshes), groups=gtab,
                 roofline=roof, errs=errs, checked=chk, total_steps=total, finite=finite,
+                schedule="parallel" if schedule == dgk.SCHEDULE_PARALLEL else "sequential",
                 sample=(meshes, phys, dts, pulses))
 
 
@@ -245,7 +260,7 @@ def main():
                                        "boxes, K~U[5k,80k], order~U{2..5}, mixed walls, own dt each), FP32 "
                                        "compensated form, groups as branches of one captured hipGraph",
                            "simulations": 256, "elements": bt["elements"], "dofs": bt["dof"],
-                           "flux_sign": "stable"},
+                           "flux_sign": "stable", "schedule": bt["schedule"]},
                    fp32_rel_l2_vs_fp64={"after_steps": bt["total_steps"], "members": bt["checked"], **bt["errs"]},
                    roofline=bt["roofline"], groups=bt["groups"], finite=bt["finite"])
     elif "fp32" in res:

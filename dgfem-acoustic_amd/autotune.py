@@ -1,0 +1,101 @@
+"""Autotuner for the stage kernel's launch shape and the batch schedule.
+
+For each (dim, order, precision) it builds a representative mesh (about `dofs` unknowns: a
+structured 2D rectangle or a 3D Kuhn box), times every candidate with HIP events around each
+stage launch (dgk_rk4_timed) and keeps the fastest:
+
+  variant     thread-per-node kernel (1, FP64 only) or MFMA kernel (2)
+  grid_pct    persistent grid as a percentage of the occupancy-derived grid: 50, 100, 200, 400
+  tile_order  XCD-aware slabs (0) or round-robin (1)
+
+The table is written as text lines "dim order precision variant grid_pct tile_order stage_us"
+(libdgk reads the first six columns, dgk_load_tuning) to dgfem-acoustic_amd/tuning.txt, which
+dgk.py loads with the library. Results never depend on the launch shape; the test suite checks
+that (tests/test_gpu_tuning.py).
+
+Usage: python dgfem-acoustic_amd/autotune.py [--dofs 4e6] [--out PATH] [--dims 2,3] [--orders 1-5]
+"""
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "tests"))
+os.environ["DGK_NO_TUNING"] = "1"  # time the untuned defaults and the candidates explicitly
+
+import dgk  # noqa: E402
+from workloads import conf  # noqa: E402
+
+GRID = (50, 100, 200, 400)
+ORDERS = (0, 1)
+
+
+def mesh_for(dim, p, dofs):
+    from math import comb
+
+    Np = comb(p + dim, dim)
+    K = max(64, int(dofs / ((dim + 1) * Np)))
+    text = conf("Wall=Reflecting\nAbs=Absorbing\n")
+    if dim == 1:
+        return dgk.gen_line(K, 1.0, p, [1, 2], "1:Wall;2:Abs", text)
+    if dim == 2:
+        n = max(2, int(round(np.sqrt(K / 2))))
+        return dgk.gen_rect(n, n, 1.0, 1.0, p, [1, 2, 1, 2], "1:Wall;2:Abs", text)
+    n = max(2, int(round(np.cbrt(K / 6))))
+    return dgk.gen_box(n, n, n, 1.0, 1.0, 1.0, p, 0.05, 1, [1, 2, 1, 2, 1, 2], "1:Wall;2:Abs", text)
+
+
+def time_case(m, precision, variant, nsteps=10):
+    s = dgk.Solver(m, 340.0, 1.225, precision=precision, flux_sign=dgk.SIGN_STABLE, variant=variant)
+    u = np.zeros((4, m.K * m.Np))
+    u[0] = np.exp(-((m.node_xyz - m.node_xyz.mean(axis=0)) ** 2).sum(axis=1) / 0.01)
+    s.set_state(u)
+    res = {}
+    for g in GRID if variant == dgk.VARIANT_MFMA else (100,):
+        for o in ORDERS if variant == dgk.VARIANT_MFMA else (0,):
+            s.set_launch(g, o)
+            s.rk4_timed(2, 1e-7)  # warm the launch shape
+            ms, _ = s.rk4_timed(nsteps, 1e-7)
+            res[(variant, g, o)] = ms * 1e3 / (4 * nsteps)
+    s.close()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--dofs", type=float, default=4e6)
+    ap.add_argument("--dims", default="1,2,3")
+    ap.add_argument("--orders", default="1,2,3,4,5")
+    ap.add_argument("--precisions", default="fp64,fp32")
+    ap.add_argument("--no-lds", action="store_true", help="skip the thread-per-node variant")
+    ap.add_argument("--out", default=os.path.join(HERE, "tuning.txt"))
+    args = ap.parse_args()
+    lines = ["# dim order precision variant grid_pct tile_order stage_us  (written by autotune.py on %s)"
+             % time.strftime("%Y-%m-%d")]
+    precs = {"fp64": dgk.FP64, "fp32": dgk.FP32}
+    for dim in [int(d) for d in args.dims.split(",")]:
+        for p in [int(o) for o in args.orders.split(",")]:
+            m = mesh_for(dim, p, args.dofs)
+            for pname in args.precisions.split(","):
+                prec = precs[pname]
+                res = time_case(m, prec, dgk.VARIANT_MFMA)
+                # the thread-per-node kernel implements the FP64 form and the plain FP32 form only;
+                # the default FP32 form (compensated solution) stays on the MFMA kernel
+                if not args.no_lds and prec == dgk.FP64:
+                    res.update(time_case(m, prec, dgk.VARIANT_LDS))
+                (v, g, o), us = min(res.items(), key=lambda kv: kv[1])
+                base = res[(dgk.VARIANT_MFMA, 100, 0)]
+                print(f"[autotune] dim={dim} p={p} {pname} K={m.K}: best variant={v} grid={g}% order={o} "
+                      f"{us:.1f} us (default {base:.1f} us, {base / us:.2f}x)", file=sys.stderr, flush=True)
+                lines.append(f"{dim} {p} {prec} {v} {g} {o} {us:.2f}")
+    with open(args.out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print(args.out)
+
+
+if __name__ == "__main__":
+    main()

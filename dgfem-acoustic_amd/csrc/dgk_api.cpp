@@ -1,6 +1,10 @@
 // C ABI of libdgk (include/dgk.h): context setup, state transfer and time stepping.
 #include <algorithm>
 #include <array>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
 #include <cmath>
 #include <cstring>
 #include <unordered_map>
@@ -204,6 +208,13 @@ void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* 
   c.sign_mode = o ? o->flux_sign : DGK_SIGN_REFERENCE;
   c.device = o ? o->device : 0;
   c.variant = o ? o->variant : 0;
+  Tuned tuned;
+  const bool have_tuning = find_tuning(m->dim, m->order, o ? o->precision : DGK_FP64, &tuned);
+  if (have_tuning) {
+    c.grid_pct = tuned.grid_pct;
+    c.tile_order = tuned.tile_order;
+    if (c.variant == 0) c.variant = tuned.variant;
+  }
   if (c.variant == 0) c.variant = kVariantMfma;
   if (c.variant != kVariantLds && c.variant != kVariantMfma) throw InvalidArg("unknown stage-kernel variant");
   const int flags = o ? o->fp32_flags : 0;
@@ -290,6 +301,7 @@ void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* 
   c.perm = (int32_t*)c.alloc(KP * c.Nf * sizeof(int32_t));
   c.flag = (int*)c.alloc(sizeof(int));
   for (void* p : {c.U, c.A, c.V0, c.V1}) DGK_HIP(hipMemsetAsync(p, 0, n * c.tsize, c.stream));
+  c.nq = c.dim + 1;  // the state starts at zero
   if (c.comp) {
     c.Ulo = (uint16_t*)c.alloc(n * sizeof(uint16_t));
     DGK_HIP(hipMemsetAsync(c.Ulo, 0, n * sizeof(uint16_t), c.stream));
@@ -334,6 +346,8 @@ namespace {
 
 StageArgs base_args(Ctx& c) {
   StageArgs a{};
+  a.nq = c.variant == kVariantLds ? 4 : c.nq;
+  a.tile_order = c.tile_order;
   a.K = c.K;
   a.Kpad = c.Kpad;
   a.PS = c.PS;
@@ -615,6 +629,26 @@ void euler_impl(Ctx& c, int nsteps, double dt, const double* ts) {
 
 }  // namespace
 
+namespace {
+std::mutex g_tune_mu;
+std::map<std::array<int, 3>, Tuned> g_tune;  // (dim, order, precision) -> launch settings
+}  // namespace
+
+bool find_tuning(int dim, int p, int precision, Tuned* t) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find({dim, p, precision});
+  if (it == g_tune.end()) return false;
+  *t = it->second;
+  return true;
+}
+
+int live_planes(int dim, const double* u, int64_t n) {
+  for (int q = dim + 1; q < 4; ++q)
+    for (int64_t i = 0; i < n; ++i)
+      if (u[q * n + i] != 0.0) return 4;
+  return dim + 1;
+}
+
 double* staging(Ctx& c) {
   if (!c.stage_d) {
     c.stage_d = (double*)c.alloc(4 * c.PS * sizeof(double));
@@ -689,6 +723,17 @@ dgk_status dgk_info_get(const dgk_ctx* c, dgk_info* info) {
 dgk_status dgk_set_state(dgk_ctx* c, const double* u) {
   if (!c || !u) return fail(c, DGK_ERR_INVALID, "NULL argument");
   return guarded(c, [&] {
+    const int nq = live_planes(c->dim, u, c->K * c->Np);
+    if (nq != c->nq && c->graph) {  // captured launches carry the plane count
+      DGK_HIP(hipGraphExecDestroy(c->graph));
+      c->graph = nullptr;
+    }
+    c->nq = nq;
+    // planes beyond nq are never written by the kernels: clear them in every buffer (Euler steps
+    // swap U with a stage buffer that may hold values from an earlier 4-plane state)
+    for (void* p : {c->U, c->A, c->V0, c->V1})
+      DGK_HIP(hipMemsetAsync((char*)p + nq * c->PS * c->tsize, 0, (4 - nq) * c->PS * c->tsize, c->stream));
+    if (c->Ulo) DGK_HIP(hipMemsetAsync(c->Ulo + nq * c->PS, 0, (4 - nq) * c->PS * sizeof(uint16_t), c->stream));
     upload_planes(*c, u, c->U);
     DGK_HIP(hipStreamSynchronize(c->stream));
   });
@@ -753,6 +798,42 @@ dgk_status dgk_get_receivers(dgk_ctx* c, double* out, int capacity_rows, int* ro
   });
 }
 
+dgk_status dgk_set_launch(dgk_ctx* c, int grid_pct, int tile_order) {
+  if (!c || grid_pct < 1 || grid_pct > 10000 || tile_order < 0 || tile_order > 1)
+    return fail(c, DGK_ERR_INVALID, "bad launch settings");
+  return guarded(c, [&] {
+    DGK_HIP(hipStreamSynchronize(c->stream));
+    c->grid_pct = grid_pct;
+    c->tile_order = tile_order;
+    if (c->graph) { DGK_HIP(hipGraphExecDestroy(c->graph)); c->graph = nullptr; }
+  });
+}
+
+dgk_status dgk_load_tuning(const char* path) {
+  if (!path) return fail(nullptr, DGK_ERR_INVALID, "NULL path");
+  return guarded(nullptr, [&] {
+    std::ifstream f(path);
+    if (!f) throw InvalidArg(std::string("cannot open tuning table ") + path);
+    std::map<std::array<int, 3>, Tuned> t;
+    std::string line;
+    while (std::getline(f, line)) {
+      if (line.empty() || line[0] == '#') continue;
+      std::istringstream ss(line);
+      int d, p, prec;
+      Tuned v;
+      if (!(ss >> d >> p >> prec >> v.variant >> v.grid_pct >> v.tile_order))
+        throw InvalidArg("malformed tuning line: " + line);
+      if (d < 1 || d > 3 || p < 1 || p > 5 || (prec != DGK_FP64 && prec != DGK_FP32) ||
+          (v.variant != kVariantLds && v.variant != kVariantMfma) || v.grid_pct < 1 || v.grid_pct > 10000 ||
+          v.tile_order < 0 || v.tile_order > 1)
+        throw InvalidArg("out-of-range tuning line: " + line);
+      t[{d, p, prec}] = v;
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune = t;
+  });
+}
+
 dgk_status dgk_rk4(dgk_ctx* c, int nsteps, double dt, const double* t_start) {
   if (!c) return fail(nullptr, DGK_ERR_INVALID, "NULL context");
   return guarded(c, [&] { rk4_impl(*c, nsteps, dt, t_start); });
@@ -772,9 +853,12 @@ dgk_status dgk_rhs(dgk_ctx* c, const double* u, double* out, double dt) {
     a.stage = 4;
     a.Vin = c->V1;
     a.Vout = c->V0;
+    a.nq = live_planes(c->dim, u, c->K * c->Np);
     launch_stage(*c, a, c->stream);
     download_planes(*c, c->V0, out);
     DGK_HIP(hipStreamSynchronize(c->stream));
+    const int64_t n = c->K * c->Np;  // planes that are not live have a zero right-hand side
+    std::fill(out + a.nq * n, out + 4 * n, 0.0);
   });
 }
 
@@ -853,6 +937,7 @@ struct dgk_batch {
   };
   std::vector<Group> groups;
   std::vector<std::pair<int, int>> where;  // sim -> (group, member)
+  std::vector<int> nq;                     // live planes of each member's state
   std::vector<int64_t> K;                  // elements per sim
   hipStream_t main = nullptr;
   hipEvent_t fork = nullptr;
@@ -891,6 +976,7 @@ void batch_create_impl(dgk_batch& B, int n, const dgk_mesh* meshes, const dgk_ph
   if (n <= 0 || !meshes || !phys || !dt) throw InvalidArg("empty batch");
   B.where.resize(n);
   B.K.resize(n);
+  B.nq.assign(n, 0);
   // Group key: dim, order, physics. A group closes when adding a member would break the per-context
   // limits (32-bit plane indexing, 2^21 vertices in 3D); the next member of that key opens a new one.
   std::vector<std::vector<int>> members;
@@ -953,6 +1039,7 @@ void batch_create_impl(dgk_batch& B, int n, const dgk_mesh* meshes, const dgk_ph
       tag_base += mx + 1;
       B.where[G.sims[k]] = {(int)B.groups.size(), (int)k};
       B.K[G.sims[k]] = m.num_elements;
+      B.nq[G.sims[k]] = m.dim + 1;
     }
     dgk_mesh um{m0.dim, m0.order, Ktot, xyz.data(), tags.data(), bc.data()};
     G.ctx = new Ctx();
@@ -1069,6 +1156,16 @@ dgk_status batch_state_io(dgk_batch* b, int sim, double* u, bool to_device) {
     DGK_HIP(hipStreamSynchronize(b->main));
     const int64_t off = b->groups[w.first].off[w.second] * c.Np, n = b->K[sim] * c.Np;
     const bool f64 = c.precision == DGK_FP64;
+    if (to_device) {  // the group's live planes: the most any member needs
+      b->nq[sim] = dgk::live_planes(c.dim, u, n);
+      int nq = c.dim + 1;
+      for (int i : b->groups[w.first].sims) nq = std::max(nq, b->nq[i]);
+      if (nq != c.nq && b->graph) {
+        DGK_HIP(hipGraphExecDestroy(b->graph));
+        b->graph = nullptr;
+      }
+      c.nq = nq;
+    }
     double* s = f64 ? nullptr : dgk::staging(c);
     for (int q = 0; q < 4; ++q) {
       const int64_t o = q * c.PS + off;
@@ -1151,6 +1248,19 @@ dgk_status dgk_batch_get_receivers(dgk_batch* b, int sim, double* out, int capac
       for (int i = 0; i < r; ++i)
         for (size_t j = 0; j < n; ++j) out[(size_t)i * n + j] = h[(size_t)i * ng + col0 + j];
     }
+  });
+}
+
+dgk_status dgk_batch_set_launch(dgk_batch* b, int group, int grid_pct, int tile_order) {
+  if (!b || group < 0 || group >= (int)b->groups.size() || grid_pct < 1 || grid_pct > 10000 || tile_order < 0 ||
+      tile_order > 1)
+    return bfail(b, DGK_ERR_INVALID, "bad launch settings");
+  return bguard(b, [&] {
+    DGK_HIP(hipStreamSynchronize(b->main));
+    dgk::Ctx& c = *b->groups[group].ctx;
+    c.grid_pct = grid_pct;
+    c.tile_order = tile_order;
+    if (b->graph) { DGK_HIP(hipGraphExecDestroy(b->graph)); b->graph = nullptr; }
   });
 }
 

@@ -73,8 +73,10 @@ struct StageArgs {
   double fsc[kMaxNF];      // reference face measure / face reference-element measure
   double dt, c0, rho0, v0[3];
   int stage;  // 0..3 RK4 stage, 4 = rhs (Vout = dt*L), 5 = forward Euler (Vout = Vin + dt*L)
+  int nq;     // field planes that carry data: dim+1 when the velocity planes beyond dim are zero, else 4
   int tiles_per_xcd;
   int64_t tiles;
+  int tile_order;  // 0: XCD-aware (XCD x sweeps a contiguous slab of tiles), 1: round-robin
 };
 
 struct Ctx {
@@ -91,6 +93,14 @@ struct Ctx {
   uint16_t* Ulo = nullptr;  // FP32 compensated form: bf16 low word of U
   bool comp = false;        // FP32: compensated solution (U, Ulo)
   bool oplo = false;        // FP32: bf16 operator-residual correction on MFMA
+  // Field planes that carry data. In 1D/2D the velocity components beyond dim stay identically
+  // zero under the dynamics when they start at zero (no derivative or normal has those
+  // components), so their planes are neither read nor written: nq = dim+1. A state with nonzero
+  // values there keeps all 4 planes live.
+  int nq = 4;
+  // launch shape (tunable): persistent grid as a percentage of the occupancy-derived grid, and
+  // the tile order
+  int grid_pct = 100, tile_order = 0;
   int32_t *nbr = nullptr, *code = nullptr, *nface = nullptr, *perm = nullptr;
   double* stage_d = nullptr;  // FP64 staging for state transfer in FP32 mode
   int* flag = nullptr;
@@ -153,6 +163,12 @@ extern thread_local std::string g_err;
 void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* o);
 void rk4_step(Ctx& c, double dt, hipStream_t s, hipEvent_t* ev = nullptr);
 double* staging(Ctx& c);
+int live_planes(int dim, const double* u, int64_t n);  // nq of a host state u[4][n]
+// tuning table: launch settings per (dim, order, precision), loaded by dgk_load_tuning
+struct Tuned {
+  int variant = 0, grid_pct = 100, tile_order = 0;
+};
+bool find_tuning(int dim, int p, int precision, Tuned* t);
 struct RefElem;
 std::vector<int64_t> locate_points(const RefElem& R, const double* node_xyz, int64_t K, int n, const double* xyz,
                                    std::vector<double>& w);

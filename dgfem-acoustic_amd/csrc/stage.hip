@@ -309,6 +309,8 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
   const T rc2 = (T)(a.rho0 * a.c0 * a.c0), irho = (T)(1.0 / a.rho0);
   const T v0[3] = {(T)a.v0[0], (T)a.v0[1], (T)a.v0[2]};
   const int stage = a.stage;
+  const int nq = a.nq;
+  for (int i = tid + nq * TE * NP; i < 4 * TE * NP; i += kBlock) (&sU[0][0])[i] = (T)0;  // planes that stay zero
   for (int i = tid; i < NF * NPERM * NFP; i += kBlock) sVmap[i] = a.vmap[i];
   for (int i = tid; i < NF * NFP; i += kBlock) sFmask[i] = a.fmask[i];
   if (kALds)
@@ -338,7 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
   auto prefetch = [&](int e0) {
 #pragma unroll
     for (int t = 0; t < NE; ++t)
-      if (tid + t * kBlock < 4 * PV) pst[t] = *(const VT*)(Vin + voff[t] + e0 * NP);
+      if (tid + t * kBlock < nq * PV) pst[t] = *(const VT*)(Vin + voff[t] + e0 * NP);
 #pragma unroll
     for (int t = 0; t < NGEO; ++t) {
       const int idx = tid + t * kBlock;
@@ -365,7 +367,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
       if (inc[t].x >= 0) {
         const int pi = inc[t].x * NP + sVmap[(inc[t].y & 0xffff) * NFP + k];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) iqp[t][c] = Vin[c * PS + pi];
+        for (int c = 0; c < 4; ++c) iqp[t][c] = c < nq ? Vin[c * PS + pi] : (T)0;
       } else {
 #pragma unroll
         for (int c = 0; c < 4; ++c) iqp[t][c] = (T)0;
@@ -373,8 +375,12 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
     }
   };
 
-  const int xcd = blockIdx.x % 8, lb = blockIdx.x / 8, bpx = gridDim.x / 8;
-  const int tpx = a.tiles_per_xcd, ntiles = (int)a.tiles;
+  // Tile walk: XCD-aware (workgroup b runs on XCD b%8; XCD x sweeps tiles [x*tpx, (x+1)*tpx), so
+  // neighbour traces are reused from that XCD's L2) or round-robin over the whole grid.
+  const bool rr = a.tile_order == 1;
+  const int xcd = rr ? 0 : blockIdx.x % 8, lb = rr ? blockIdx.x : blockIdx.x / 8;
+  const int bpx = rr ? gridDim.x : gridDim.x / 8;
+  const int tpx = rr ? (int)a.tiles : a.tiles_per_xcd, ntiles = (int)a.tiles;
   auto valid = [&](int it) { return it < tpx && xcd * tpx + it < ntiles; };
   __syncthreads();  // sVmap ready for the first gathers
   if (valid(lb)) {
@@ -387,7 +393,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
     // phase 1: commit the prefetched tile to LDS; epilogue operands and the next tile in flight
 #pragma unroll
     for (int t = 0; t < NE; ++t)
-      if (tid + t * kBlock < 4 * PV) *(VT*)(&sU[0][0] + loff[t]) = pst[t];
+      if (tid + t * kBlock < nq * PV) *(VT*)(&sU[0][0] + loff[t]) = pst[t];
 #pragma unroll
     for (int t = 0; t < NGEO; ++t) {
       const int idx = tid + t * kBlock;
@@ -405,7 +411,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
     if (stage >= 1 && stage <= 3) {
 #pragma unroll
       for (int t = 0; t < NE; ++t)
-        if (tid + t * kBlock < 4 * PV) {
+        if (tid + t * kBlock < nq * PV) {
           const int id = voff[t] + e0 * NP;
           if (stage != 3 || CS) pu[t] = *(const VT*)(U + id);
           if (CS && stage == 3) plo[CS ? t : 0] = *(const uint2*)(Ulo + id);
@@ -518,7 +524,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
     const T s6 = (T)(1.0 / 6.0), s3 = (T)(1.0 / 3.0), s2 = (T)0.5;
 #pragma unroll
     for (int t = 0; t < NE; ++t) {
-      if (tid + t * kBlock >= 4 * PV) break;
+      if (tid + t * kBlock >= nq * PV) break;
       const VT kq = dt * *(const VT*)(sR + loff[t]);
       const int id = voff[t] + e0 * NP;
       switch (stage) {
@@ -649,7 +655,7 @@ void launch_lds(StageArgs& a, hipStream_t s) {
 }
 
 template <typename T, int D, int P, bool LO, bool CS>
-void launch_mfma(StageArgs& a, hipStream_t s) {
+void launch_mfma(StageArgs& a, hipStream_t s, int grid_pct) {
   constexpr int TE = MfmaShape<D, P>::TE;
   static int occ = 0;
   if (!occ) {
@@ -659,8 +665,8 @@ void launch_mfma(StageArgs& a, hipStream_t s) {
   }
   a.tiles = a.Kpad / TE;
   a.tiles_per_xcd = (int)((a.tiles + 7) / 8);
-  int64_t per_xcd_cap = (int64_t)(num_cus() / 8) * occ;
-  int bpx = (int)std::min<int64_t>(a.tiles_per_xcd, per_xcd_cap);
+  const int64_t per_xcd_cap = (int64_t)(num_cus() / 8) * occ * std::max(grid_pct, 1) / 100;
+  int bpx = (int)std::min<int64_t>(a.tiles_per_xcd, std::max<int64_t>(per_xcd_cap, 1));
   if (bpx < 1) bpx = 1;
   hipLaunchKernelGGL((k_stage_mfma<T, D, P, LO, CS>), dim3(8 * bpx), dim3(kBlock), 0, s, a);
 }
@@ -669,11 +675,11 @@ template <typename T, int D, int P>
 void launch_variant(Ctx& c, StageArgs& a, hipStream_t s) {
   if (c.variant == kVariantLds) return launch_lds<T, D, P>(a, s);
   if constexpr (std::is_same<T, float>::value) {
-    if (c.comp && c.oplo) return launch_mfma<T, D, P, true, true>(a, s);
-    if (c.comp) return launch_mfma<T, D, P, false, true>(a, s);
-    if (c.oplo) return launch_mfma<T, D, P, true, false>(a, s);
+    if (c.comp && c.oplo) return launch_mfma<T, D, P, true, true>(a, s, c.grid_pct);
+    if (c.comp) return launch_mfma<T, D, P, false, true>(a, s, c.grid_pct);
+    if (c.oplo) return launch_mfma<T, D, P, true, false>(a, s, c.grid_pct);
   }
-  launch_mfma<T, D, P, false, false>(a, s);
+  launch_mfma<T, D, P, false, false>(a, s, c.grid_pct);
 }
 
 template <typename T>

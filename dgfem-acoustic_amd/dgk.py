@@ -15,6 +15,8 @@ import numpy as np
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 HOST_LIB = os.path.join(PKG_DIR, "libdgkhost.so")
 DEV_LIB = os.environ.get("DGK_LIBDGK") or os.path.join(PKG_DIR, "libdgk.so")
+# launch settings per (dim, order, precision) written by autotune.py; loaded with the library
+TUNING_FILE = os.environ.get("DGK_TUNING") or os.path.join(PKG_DIR, "tuning.txt")
 
 _host = None
 _dev = None
@@ -190,7 +192,8 @@ DEV_SYMBOLS = ("dgk_create", "dgk_destroy", "dgk_last_error", "dgk_info_get", "d
                "dgk_connectivity", "dgk_rk4_timed", "dgk_variant_name", "dgk_batch_create", "dgk_batch_destroy",
                "dgk_batch_last_error", "dgk_batch_set_state", "dgk_batch_get_state", "dgk_batch_rk4",
                "dgk_batch_sync", "dgk_batch_groups", "dgk_batch_prepare", "dgk_batch_rk4_timed",
-               "dgk_set_receivers", "dgk_get_receivers", "dgk_batch_set_receivers", "dgk_batch_get_receivers")
+               "dgk_set_receivers", "dgk_get_receivers", "dgk_batch_set_receivers", "dgk_batch_get_receivers",
+               "dgk_set_launch", "dgk_load_tuning", "dgk_batch_set_launch")
 
 
 def device():
@@ -231,6 +234,12 @@ def device():
         L.dgk_batch_set_receivers.argtypes = [vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double),
                                               ctypes.c_int]
         L.dgk_batch_get_receivers.argtypes = [vp, ctypes.c_int, P(ctypes.c_double), ctypes.c_int, P(ctypes.c_int)]
+        L.dgk_set_launch.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.dgk_load_tuning.argtypes = [ctypes.c_char_p]
+        L.dgk_batch_set_launch.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        if os.path.exists(TUNING_FILE) and not os.environ.get("DGK_NO_TUNING"):
+            if L.dgk_load_tuning(TUNING_FILE.encode()):
+                raise RuntimeError(f"tuning table {TUNING_FILE}: {L.dgk_last_error(None).decode()}")
         L.dgk_batch_prepare.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.dgk_batch_rk4_timed.argtypes = [vp, ctypes.c_int, P(ctypes.c_double)]
         L.dgk_batch_groups.argtypes = [vp, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int64),
@@ -335,6 +344,10 @@ class Solver:
         self._check(L.dgk_get_receivers(self.h, _p(out), rows.value, ctypes.byref(rows)))
         return out
 
+    def set_launch(self, grid_pct=100, tile_order=0):
+        """Launch shape of the stage kernel (tuning only; results are unchanged)."""
+        self._check(device().dgk_set_launch(self.h, int(grid_pct), int(tile_order)))
+
     def rk4_timed(self, nsteps, dt):
         s, t = ctypes.c_double(), ctypes.c_double()
         self._check(device().dgk_rk4_timed(self.h, int(nsteps), float(dt), ctypes.byref(s), ctypes.byref(t)))
@@ -425,6 +438,9 @@ class Batch:
         out = np.zeros((rows.value, getattr(self, "_nrec", {}).get(i, 0)))
         self._check(L.dgk_batch_get_receivers(self.h, int(i), _p(out), rows.value, ctypes.byref(rows)))
         return out
+
+    def set_launch(self, group, grid_pct=100, tile_order=0):
+        self._check(device().dgk_batch_set_launch(self.h, int(group), int(grid_pct), int(tile_order)))
 
     def prepare(self, nsteps, schedule=SCHEDULE_PARALLEL):
         """Capture the graph for rk4(nsteps, schedule) ahead of time."""

@@ -16,6 +16,8 @@
 #include <string>
 #include <vector>
 
+#include <unistd.h>
+
 #include "../../include/dgk.h"
 #include "../../include/dgk_host.h"
 #include "config.hpp"
@@ -101,6 +103,21 @@ int main(int argc, char** argv) {
   op.precision = cfg.precision == "fp32" ? DGK_FP32 : DGK_FP64;
   op.flux_sign = cfg.fluxSign == "stable" ? DGK_SIGN_STABLE : DGK_SIGN_REFERENCE;
   op.device = cfg.device;
+  // launch settings tuned for this machine: $DGK_TUNING, else tuning.txt next to the executable
+  {
+    std::string tpath;
+    if (const char* e = std::getenv("DGK_TUNING")) tpath = e;
+    else {
+      char buf[4096];
+      const ssize_t len = readlink("/proc/self/exe", buf, sizeof(buf) - 1);
+      if (len > 0) {
+        std::string exe(buf, (size_t)len);
+        tpath = exe.substr(0, exe.find_last_of('/') + 1) + "tuning.txt";
+      }
+    }
+    if (!tpath.empty() && access(tpath.c_str(), R_OK) == 0 && dgk_load_tuning(tpath.c_str()) != DGK_OK)
+      log("ignoring tuning table " + tpath + ": " + dgk_last_error(nullptr));
+  }
   dgk_ctx* ctx = nullptr;
   if (dgk_create(&m, &ph, &op, &ctx) != DGK_OK) return die(std::string("libdgk: ") + dgk_last_error(nullptr));
   auto check = [&](dgk_status st) {

@@ -190,9 +190,9 @@ class Surrogate:
         x = self.encode(src, rcv)
         t = np.asarray(series, np.float64) / self.scale
         # relative L2 loss per sample; pairs the pulse has not reached within the window carry
-        # almost no signal, so their weight is capped at that of a series 1% of the median norm
+        # almost no signal, so their weight is capped at that of a series 1% of the largest norm
         nrm2 = (t * t).sum(axis=1)
-        w = 1.0 / np.maximum(nrm2, 1e-4 * np.median(nrm2))
+        w = 1.0 / np.maximum(nrm2, 1e-4 * nrm2.max())
         out = []
         for k, n in enumerate(self.nets):
             n.set_data(x, t, w)

@@ -123,6 +123,14 @@ dgk_status dgk_check_finite(dgk_ctx* ctx);
 dgk_status dgk_set_receivers(dgk_ctx* ctx, int n, const double* xyz, const double* node_xyz, int capacity);
 /* rows = steps recorded; out (may be NULL to query rows) receives rows x n values, row-major */
 dgk_status dgk_get_receivers(dgk_ctx* ctx, double* out, int capacity_rows, int* rows);
+/* Launch shape of the stage kernel (no reference counterpart; tuning): the persistent grid as a
+ * percentage of the occupancy-derived grid (100 = one full wave of resident workgroups) and the
+ * tile order (0 = XCD-aware slabs, 1 = round-robin). Results do not depend on either. */
+dgk_status dgk_set_launch(dgk_ctx* ctx, int grid_pct, int tile_order);
+/* Load a tuning table (text lines "dim order precision variant grid_pct tile_order", '#'
+ * comments), applied to contexts created afterwards: launch shape always, variant when
+ * dgk_opts.variant is 0. Replaces any table loaded before. */
+dgk_status dgk_load_tuning(const char* path);
 dgk_status dgk_connectivity(dgk_ctx* ctx, int32_t* nbr, int32_t* nface, int32_t* perm, int32_t* sign);
 
 /* Timed run for measurement: nsteps RK4 steps with a HIP event pair around every stage
@@ -153,6 +161,8 @@ dgk_status dgk_batch_rk4(dgk_batch* b, int nsteps, int schedule);
 dgk_status dgk_batch_set_receivers(dgk_batch* b, int sim, int n, const double* xyz, const double* node_xyz,
                                    int capacity);
 dgk_status dgk_batch_get_receivers(dgk_batch* b, int sim, double* out, int capacity_rows, int* rows);
+/* launch shape of one group's stage kernel (see dgk_set_launch) */
+dgk_status dgk_batch_set_launch(dgk_batch* b, int group, int grid_pct, int tile_order);
 /* capture (and keep) the graph that dgk_batch_rk4(b, nsteps, schedule) will replay, without running it */
 dgk_status dgk_batch_prepare(dgk_batch* b, int nsteps, int schedule);
 /* nsteps RK4 steps group by group without a graph, with a HIP event pair around every stage

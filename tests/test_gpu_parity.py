@@ -238,3 +238,47 @@ def test_fp32_euler_and_sources_compensated():
     ref = o.step(ref, 10, dt, t_steps=ts[:10] + 60 * dt, sources=src, euler=True)
     ref = o.step(ref, 30, dt, t_steps=ts[:30] + 70 * dt, sources=src)
     assert rel(s.get_state()[:3], ref[:3]) <= 1e-5
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_zero_velocity_planes_stay_zero_and_switch(dim):
+    """1D/2D states whose extra velocity components are zero run on dim+1 planes; a later state
+    with nonzero extra components (and back) gives the oracle's result either way."""
+    m = mixed_mesh(dim, 3)
+    o = ob.Oracle(m, 340.0, 1.225, v0=(20.0, 0.0, 0.0), sign_mode=ob.SIGN_STABLE)
+    s = dgk.Solver(m, 340.0, 1.225, v0=(20.0, 0.0, 0.0), flux_sign=dgk.SIGN_STABLE)
+    rng = np.random.default_rng(3)
+    full = rng.standard_normal((4, m.K * m.Np))
+    live = full.copy()
+    live[dim + 1:] = 0.0
+    for u in (live, full, live):
+        s.set_state(u)
+        s.euler(3, 2e-6)
+        s.rk4(30, 2e-6)
+        ref = o.step(o.step(u, 3, 2e-6, euler=True), 30, 2e-6)
+        g = s.get_state()
+        assert rel(g, ref) <= 1e-12
+        if u is live:
+            assert not g[dim + 1:].any()
+        assert rel(s.rhs(u, 1e-4), o.rhs(u, 1e-4)) <= 1e-13
+
+
+def test_room_full_size_fp32_vs_fp64_with_receivers():
+    """The 20 x 15 x 8 m room with its block obstacle at order 4 (920,064 tets, 129 M unknowns):
+    the FP32 form against the FP64 drop-in after 20 steps, fields and 8 receivers."""
+    m, ph, dt, u0 = wl.room_3d()
+    assert m.K == 920064 and m.Np == 35
+    rng = np.random.default_rng(6)
+    rec = np.column_stack([20 * rng.random(8), 15 * rng.random(8), 1.5 + 6 * rng.random(8)])
+    out = {}
+    for prec in (dgk.FP64, dgk.FP32):
+        s = dgk.Solver(m, ph["c0"], ph["rho0"], precision=prec, flux_sign=dgk.SIGN_STABLE)
+        s.set_state(u0)
+        s.set_receivers(rec, 20)
+        s.rk4(20, dt)
+        out[prec] = (s.get_state(), s.get_receivers())
+        assert s.check_finite()
+        s.close()
+    g64, g32 = out[dgk.FP64][0], out[dgk.FP32][0]
+    assert rel(g32[0], g64[0]) <= 1e-5
+    assert rel(out[dgk.FP32][1], out[dgk.FP64][1]) <= 1e-4
