@@ -180,8 +180,8 @@ def batch_leg(steps, warmup, members_checked=8, n=256):
                 sample=(meshes, phys, dts, pulses))
 
 
-def cpu_baseline(meshes, phys, dts, pulses, members):
-    """Literal (reference loop structure) oracle on a bounded sample: one RK4 step of each member."""
+def cpu_baseline(meshes, phys, dts, pulses, members, steps=2):
+    """Literal (reference loop structure) oracle on a bounded sample: `steps` RK4 steps of each member."""
     import oracle_binding as ob
 
     threads = os.cpu_count() or 1
@@ -192,17 +192,18 @@ def cpu_baseline(meshes, phys, dts, pulses, members):
         u = wl.ensemble_state(m, pulses[i])
         o = ob.Oracle(m, ph["c0"], ph["rho0"], v0=ph["v0"], sign_mode=ob.SIGN_STABLE, nthreads=threads)
         t0 = time.perf_counter()
-        o.step(u, 1, dts[i], mode=ob.LITERAL)
+        o.step(u, steps, dts[i], mode=ob.LITERAL)
         t_lit += time.perf_counter() - t0
         t0 = time.perf_counter()
-        o.step(u, 1, dts[i], mode=ob.FACTORIZED)
+        o.step(u, steps, dts[i], mode=ob.FACTORIZED)
         t_fac += time.perf_counter() - t0
         del o
-        dof_total += (m.dim + 1) * m.K * m.Np
+        dof_total += (m.dim + 1) * m.K * m.Np * steps
     desc = ", ".join(f"member {i} ({meshes[i].dim}D p{meshes[i].order} K={meshes[i].K})" for i in members)
     return {"value": dof_total * 4 / t_lit, "unit": UNIT, "cores": threads, "kind": "port",
-            "sample": f"1 RK4 step each of {desc} with the oracle's literal path (the reference's loop "
-                      f"structure and OpenMP layout), {t_lit:.1f} s",
+            "sample": f"{steps} RK4 steps each of {desc} with the oracle's literal path (the reference's "
+                      f"loop structure and OpenMP layout), {t_lit:.1f} s; factorized_value: the oracle's "
+                      f"factorized path on the same sample",
             "factorized_value": dof_total * 4 / t_fac}
 
 
@@ -286,7 +287,7 @@ def main():
     out["cpu_baseline"] = None
     if not args.no_cpu and "batch" in res:
         log("cpu baseline: literal oracle on a sample of the ensemble, all host threads")
-        out["cpu_baseline"] = cpu_baseline(*res["batch"]["sample"], members=[0, 134])
+        out["cpu_baseline"] = cpu_baseline(*res["batch"]["sample"], members=[0, 1, 6, 134, 135])
     print(json.dumps(out))
 
 

@@ -406,8 +406,11 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
 #pragma unroll
         for (int c = 0; c < 4; ++c) sN[idx][c] = pfg[t][c];
     }
+    // epilogue operands: issued here, a tile ahead of their use, when registers allow; shapes with
+    // the operator in global memory load them just before the epilogue instead
     VT pu[NE], pa[NE];
     uint2 plo[CS ? NE : 1];
+    auto load_epi = [&]() {
     if (stage >= 1 && stage <= 3) {
 #pragma unroll
       for (int t = 0; t < NE; ++t)
@@ -418,6 +421,8 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
           pa[t] = *(const VT*)(Acc + id);
         }
     }
+    };
+    if (kALds) load_epi();
 #pragma unroll
     for (int t = 0; t < NI; ++t) cnc[t] = inc[t];
     const bool more = valid(it + bpx);
@@ -479,9 +484,14 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) acc[mb] = V{0, 0, 0, 0};
     bf16x8 blo;
+    // Fragments read from global memory (shapes whose operator does not fit in LDS) are reloaded per
+    // tile through an opaque pointer and the node loop is only partly unrolled: otherwise the
+    // compiler hoists every fragment of the operator into registers and spills.
+    const T* pAt = pA;
+    if (!kALds) asm volatile("" : "+v"(pAt));
     auto step = [&](const int s, const T b) {
 #pragma unroll
-      for (int mb = 0; mb < MB; ++mb) acc[mb] = Mfma<T>::mma(pA[(mb * KS + s) * 64], b, acc[mb]);
+      for (int mb = 0; mb < MB; ++mb) acc[mb] = Mfma<T>::mma(pAt[(mb * KS + s) * 64], b, acc[mb]);
       if constexpr (LO) {
         blo[s % 8] = (__bf16)b;
         if (s % 8 == 7 || s == KS - 1) {
@@ -493,7 +503,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
         }
       }
     };
-#pragma unroll
+#pragma unroll (kALds ? NG : 2)
     for (int m = 0; m < NG; ++m) {
       const int n = 4 * m + g;
       T p = 0, vx = 0, vy = 0, vz = 0;
@@ -520,6 +530,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
         if (i < NP) sR[q * TE * NP + el * NP + i] = acc[mb][j];
       }
     __syncthreads();
+    if (!kALds) load_epi();
     // phase 4: time-integration epilogue on 16-byte vectors
     const T s6 = (T)(1.0 / 6.0), s3 = (T)(1.0 / 3.0), s2 = (T)0.5;
 #pragma unroll
