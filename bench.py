@@ -71,11 +71,33 @@ def algorithmic_bytes(dim, p, K, F, Fb, word, compensated):
     return dof * state + traces + geo
 
 
-def roofline(bytes_per_launch, stage_s, kernel):
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summaries committed under profiles/ (calibrated against a device copy there), or None."""
+    d = os.path.join(ROOT, "profiles")
+    key = kernel.split(">")[0].replace(" ", "")
+    if not os.path.isdir(d):
+        return None
+    for f in sorted(os.listdir(d)):
+        if not f.endswith(".md"):
+            continue
+        for line in open(os.path.join(d, f)):
+            cols = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cols) == 8 and cols[0].replace(" ", "").startswith(key + ",") and cols[1] == workload:
+                return {"bytes": int(float(cols[6])), "source": "profiles/" + f}
+    return None
+
+
+def roofline(bytes_per_launch, stage_s, kernel, workload):
     ach = bytes_per_launch / stage_s
-    return {"bound": "hbm", "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK, 4), "traffic": None, "kernel": kernel,
-            "kernel_avg_us": round(stage_s * 1e6, 2), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+    tr = measured_traffic(kernel, workload)
+    out = {"bound": "hbm", "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+           "frac": round(ach / HBM_PEAK, 4), "traffic": tr["bytes"] if tr else None, "kernel": kernel,
+           "kernel_avg_us": round(stage_s * 1e6, 2), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+    if tr:
+        out["traffic_source"] = tr["source"]
+        out["traffic_GBps"] = round(tr["bytes"] / stage_s / 1e9, 1)
+    return out
 
 
 def single_leg(m, ph, dt, u0, precision, steps, warmup):
@@ -97,7 +119,8 @@ def single_leg(m, ph, dt, u0, precision, steps, warmup):
     b = algorithmic_bytes(m.dim, m.order, m.K, info.num_faces, info.num_boundary_faces, word, precision == dgk.FP32)
     stage_s = stage_ms * 1e-3 / (4 * nk)
     return dict(rate=dof * 4 * steps / wall, wall=wall, state=state, total_steps=warmup + steps + nk,
-                roofline=roofline(b, stage_s, "k_stage_mfma<%s,3,3>" % ("double" if word == 8 else "float")),
+                roofline=roofline(b, stage_s, "k_stage_mfma<%s,3,3>" % ("double" if word == 8 else "float"),
+                                  "cube3d-p3"),
                 finite=bool(np.isfinite(state).all()))
 
 
@@ -152,7 +175,7 @@ def batch_leg(steps, warmup, members_checked=8, n=256):
         x["GBps"] = round(x["bytes"] / x["stage_s"] / 1e9, 1)
     dom = max(gtab, key=lambda x: x["stage_s"])
     roof = roofline(dom["bytes"], dom["stage_s"],
-                    "k_stage_mfma<float,%d,%d> (largest batch group)" % (dom["dim"], dom["order"]))
+                    "k_stage_mfma<float,%d,%d> (largest batch group)" % (dom["dim"], dom["order"]), "ensemble256")
     roof["sum_of_group_stage_us"] = round(tot * 1e6, 1)
     roof["all_groups_GBps"] = round(sum(x["bytes"] for x in gtab) / tot / 1e9, 1)
     # FP32 error on a subset against FP64 runs of the same members
