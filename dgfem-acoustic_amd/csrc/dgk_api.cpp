@@ -198,8 +198,9 @@ void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* 
   if (m->dim < 1 || m->dim > 3) throw Unsupported("element dimension must be 1, 2 or 3");
   if (m->order < 1 || m->order > 5) throw Unsupported("element order must be 1..5");
   if (m->num_elements <= 0 || !m->node_xyz || !m->node_tags) throw InvalidArg("empty mesh");
-  if (4 * (m->num_elements + kTilePad) * num_nodes(m->dim, m->order) >= ((int64_t)1 << 31))
-    throw Unsupported("4*K*Np must stay below 2^31 (32-bit device indexing)");
+  const int64_t tsz = (o && o->precision == DGK_FP32) ? 4 : 8;
+  if (4 * (m->num_elements + kTilePad) * num_nodes(m->dim, m->order) * tsz >= ((int64_t)1 << 32))
+    throw Unsupported("a state buffer (4*K*Np values) must stay below 4 GiB (32-bit device offsets)");
   if (!(ph->c0 > 0) || !(ph->rho0 > 0)) throw InvalidArg("c0 and rho0 must be positive");
   c.dim = m->dim;
   c.p = m->order;
@@ -978,10 +979,11 @@ void batch_create_impl(dgk_batch& B, int n, const dgk_mesh* meshes, const dgk_ph
   B.K.resize(n);
   B.nq.assign(n, 0);
   // Group key: dim, order, physics. A group closes when adding a member would break the per-context
-  // limits (32-bit plane indexing, 2^21 vertices in 3D); the next member of that key opens a new one.
+  // limits (4 GiB per state buffer, 2^21 vertices in 3D); the next member of that key opens a new one.
   std::vector<std::vector<int>> members;
   std::vector<std::array<double, 7>> keys;
   std::vector<int64_t> gK, gV;
+  const int64_t tsz = (opts && opts->precision == DGK_FP32) ? 4 : 8;
   for (int i = 0; i < n; ++i) {
     const dgk_mesh& m = meshes[i];
     if (!(dt[i] > 0)) throw InvalidArg("batch time steps must be positive");
@@ -999,7 +1001,7 @@ void batch_create_impl(dgk_batch& B, int n, const dgk_mesh* meshes, const dgk_ph
                             phys[i].v0[0], phys[i].v0[1], phys[i].v0[2]};
     size_t gi = 0;
     while (gi < keys.size() &&
-           (keys[gi] != k || 4 * (gK[gi] + m.num_elements + kTilePad) * Np >= ((int64_t)1 << 31) ||
+           (keys[gi] != k || 4 * (gK[gi] + m.num_elements + kTilePad) * Np * tsz >= ((int64_t)1 << 32) ||
             (m.dim == 3 && gV[gi] + nv >= ((int64_t)1 << 21))))
       ++gi;
     if (gi == keys.size()) {

@@ -258,6 +258,18 @@ __device__ __forceinline__ int d_row<double>(int lane, int j) { return (lane >> 
 // instead of u + dt/6 k1), and stage 3 adds it to (U, Ulo) with an exact two-sum, so the
 // per-step rounding of u (2^-24, accumulating over the run) is removed; stage inputs u + c k
 // are still formed from U alone, which costs only a non-accumulating 2^-24 perturbation.
+// Element access through a uniform base pointer plus a 32-bit byte offset, which maps to the
+// global_load/store form with an SGPR base and a 32-bit VGPR offset (no 64-bit address arithmetic
+// per access). Offsets stay below 2^32: a context holds 4*PS*sizeof(T) < 2^32 bytes per buffer.
+template <typename P>
+__device__ __forceinline__ P* byte_at(P* base, uint32_t off) {
+  return (P*)((char*)base + off);
+}
+template <typename P>
+__device__ __forceinline__ const P* byte_at(const P* base, uint32_t off) {
+  return (const P*)((const char*)base + off);
+}
+
 __device__ __forceinline__ float bf16_bits_to_f32(uint32_t b) { return __uint_as_float(b << 16); }
 __device__ __forceinline__ uint32_t f32_to_bf16_bits(float f) {
   uint32_t u = __float_as_uint(f);
@@ -296,7 +308,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, col = lane & 15, el = wave * 4 + (col >> 2), q = col & 3;
-  const int PS = (int)a.PS;  // plane stride: Kpad*Np, 4*PS < 2^31 is checked at context creation
+  const int PS = (int)a.PS;  // plane stride Kpad*Np; 4*PS*sizeof(T) < 2^32 is checked at context creation
   const T* __restrict__ Vin = (const T*)a.Vin;
   T* __restrict__ Vout = (T*)a.Vout;
   T* __restrict__ U = (T*)a.U;
@@ -321,12 +333,12 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
 
   // Per-thread offsets that do not depend on the tile (every tile is full: the element count is
   // padded to a multiple of TE and padding elements carry zero state and geometry).
-  int voff[NE];  // plane*PS + r for the thread's state vectors (add e0*NP)
-  int loff[NE];  // plane*TE*NP + r in LDS
+  uint32_t voff[NE];  // plane*PS + r for the thread's state vectors (add e0*NP), in elements
+  int loff[NE];       // plane*TE*NP + r in LDS
 #pragma unroll
   for (int t = 0; t < NE; ++t) {
     const int v = tid + t * kBlock, qq = v / PV, r = (v - qq * PV) * VW;
-    voff[t] = qq * PS + r;
+    voff[t] = (uint32_t)(qq * PS + r);
     loff[t] = qq * TE * NP + r;
   }
 
@@ -340,24 +352,25 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
   auto prefetch = [&](int e0) {
 #pragma unroll
     for (int t = 0; t < NE; ++t)
-      if (tid + t * kBlock < nq * PV) pst[t] = *(const VT*)(Vin + voff[t] + e0 * NP);
+      if (tid + t * kBlock < nq * PV) pst[t] = *byte_at((const VT*)Vin, (voff[t] + (uint32_t)(e0 * NP)) * (uint32_t)sizeof(T));
 #pragma unroll
     for (int t = 0; t < NGEO; ++t) {
       const int idx = tid + t * kBlock;
-      if (idx < TE * 9) pgeo[t] = Jinv[e0 * 9 + idx];
+      if (idx < TE * 9) pgeo[t] = *byte_at(Jinv, (uint32_t)(e0 * 9 + idx) * (uint32_t)sizeof(T));
     }
 #pragma unroll
     for (int t = 0; t < NFC; ++t) {
       const int idx = tid + t * kBlock;
       if (idx < TE * NF)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) pfg[t][c] = Fg[(e0 * NF + idx) * 4 + c];
+        for (int c = 0; c < 4; ++c) pfg[t][c] = *byte_at(Fg, (uint32_t)((e0 * NF + idx) * 4 + c) * (uint32_t)sizeof(T));
     }
 #pragma unroll
     for (int t = 0; t < NI; ++t) {
       const int idx = tid + t * kBlock;
       const int ej = e0 * NF + (idx / NFF) * NF + (idx % NFF) / NFP;
-      inc[t] = idx < TE * NFF ? make_int2(a.nbr[ej], a.code[ej]) : make_int2(-1, 0);
+      inc[t] = idx < TE * NFF ? make_int2(*byte_at(a.nbr, (uint32_t)ej * 4u), *byte_at(a.code, (uint32_t)ej * 4u))
+                              : make_int2(-1, 0);
     }
   };
   auto gather = [&]() {
@@ -367,7 +380,8 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
       if (inc[t].x >= 0) {
         const int pi = inc[t].x * NP + sVmap[(inc[t].y & 0xffff) * NFP + k];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) iqp[t][c] = c < nq ? Vin[c * PS + pi] : (T)0;
+        for (int c = 0; c < 4; ++c)
+          iqp[t][c] = c < nq ? *byte_at(Vin, (uint32_t)(c * PS + pi) * (uint32_t)sizeof(T)) : (T)0;
       } else {
 #pragma unroll
         for (int c = 0; c < 4; ++c) iqp[t][c] = (T)0;
@@ -415,10 +429,10 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
 #pragma unroll
       for (int t = 0; t < NE; ++t)
         if (tid + t * kBlock < nq * PV) {
-          const int id = voff[t] + e0 * NP;
-          if (stage != 3 || CS) pu[t] = *(const VT*)(U + id);
-          if (CS && stage == 3) plo[CS ? t : 0] = *(const uint2*)(Ulo + id);
-          pa[t] = *(const VT*)(Acc + id);
+          const uint32_t id = voff[t] + (uint32_t)(e0 * NP), ib = id * (uint32_t)sizeof(T);
+          if (stage != 3 || CS) pu[t] = *byte_at((const VT*)U, ib);
+          if (CS && stage == 3) plo[CS ? t : 0] = *byte_at((const uint2*)Ulo, id * 2u);
+          pa[t] = *byte_at((const VT*)Acc, ib);
         }
     }
     };
@@ -537,15 +551,19 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
     for (int t = 0; t < NE; ++t) {
       if (tid + t * kBlock >= nq * PV) break;
       const VT kq = dt * *(const VT*)(sR + loff[t]);
-      const int id = voff[t] + e0 * NP;
+      const uint32_t id = voff[t] + (uint32_t)(e0 * NP), ib = id * (uint32_t)sizeof(T);
+      VT* const pAcc = byte_at((VT*)Acc, ib);
+      VT* const pOut = byte_at((VT*)Vout, ib);
+      VT* const pU = byte_at((VT*)U, ib);
+      uint2* const pLo = byte_at((uint2*)Ulo, id * 2u);
       switch (stage) {
         case 0: {
           const VT u = *(const VT*)(&sU[0][0] + loff[t]);
-          *(VT*)(Acc + id) = CS ? kq * s6 : u + kq * s6;
-          *(VT*)(Vout + id) = u + kq * s2;
+          *pAcc = CS ? kq * s6 : u + kq * s6;
+          *pOut = u + kq * s2;
         } break;
-        case 1: *(VT*)(Acc + id) = pa[t] + kq * s3; *(VT*)(Vout + id) = pu[t] + kq * s2; break;
-        case 2: *(VT*)(Acc + id) = pa[t] + kq * s3; *(VT*)(Vout + id) = pu[t] + kq; break;
+        case 1: *pAcc = pa[t] + kq * s3; *pOut = pu[t] + kq * s2; break;
+        case 2: *pAcc = pa[t] + kq * s3; *pOut = pu[t] + kq; break;
         case 3:
           if constexpr (CS) {
             const VT d = pa[t] + kq * s6;  // dt/6 (k1 + 2 k2 + 2 k3 + k4)
@@ -563,16 +581,16 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
               hi[i] = h2;
               ob[i] = f32_to_bf16_bits(lo - (h2 - sum));
             }
-            *(VT*)(U + id) = hi;
-            *(uint2*)(Ulo + id) = make_uint2(ob[0] | (ob[1] << 16), ob[2] | (ob[3] << 16));
+            *pU = hi;
+            *pLo = make_uint2(ob[0] | (ob[1] << 16), ob[2] | (ob[3] << 16));
           } else {
-            *(VT*)(U + id) = pa[t] + kq * s6;
+            *pU = pa[t] + kq * s6;
           }
           break;
-        case 4: *(VT*)(Vout + id) = kq; break;
+        case 4: *pOut = kq; break;
         default:
-          *(VT*)(Vout + id) = *(const VT*)(&sU[0][0] + loff[t]) + kq;
-          if (CS) *(uint2*)(Ulo + id) = make_uint2(0, 0);  // Vout becomes U: an Euler step drops the low word
+          *pOut = *(const VT*)(&sU[0][0] + loff[t]) + kq;
+          if (CS) *pLo = make_uint2(0, 0);  // Vout becomes U: an Euler step drops the low word
           break;
       }
     }
