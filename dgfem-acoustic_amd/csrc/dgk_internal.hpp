@@ -32,7 +32,7 @@ struct InvalidArg : std::runtime_error {
 };
 
 constexpr int kMaxNF = 4;
-constexpr int kTilePad = 16;  // device arrays hold a multiple of this many elements
+constexpr int kTilePad = 32;  // device arrays hold a multiple of this many elements (the widest tile)
 // stage-kernel variants (dgk_opts.variant; 0 picks the default)
 constexpr int kVariantLds = 1;   // thread per element node, operators read from memory
 constexpr int kVariantMfma = 2;  // 16x16x4 MFMA contraction, one wave per 4 elements

@@ -234,7 +234,6 @@ struct MfmaShape {
   static constexpr int NFK = (NFF + 3) / 4;   // face k-steps
   static constexpr int KS = D * NG + NFK;     // k-steps
   static constexpr int MB = (NP + 15) / 16;   // 16-row blocks
-  static constexpr int TE = 16;               // elements per workgroup tile
   static constexpr int RB = NFF > NP ? NFF : NP;
   static constexpr int NLO = (KS + 7) / 8;    // bf16 low-order correction MFMAs per m-block
 };
@@ -276,13 +275,25 @@ __device__ __forceinline__ uint32_t f32_to_bf16_bits(float f) {
   return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;  // round to nearest even (finite inputs)
 }
 
+// Workgroup shape: 4 waves (16-element tiles) by default. FP32 3D order 5 keeps its 64.5 KB
+// operator in LDS, which leaves room for one workgroup per CU; it runs 8 waves (32-element tiles)
+// so that two waves per SIMD share that one copy.
+template <typename T, int D, int P>
+struct MfmaLaunch {
+  static constexpr int WAVES = (sizeof(T) == 4 && D == 3 && P == 5) ? 8 : 4;
+  static constexpr int KB = 64 * WAVES;  // threads per workgroup
+  static constexpr int TE = 4 * WAVES;   // elements per tile
+};
+
 template <typename T, int D, int P, bool LO, bool CS>
-__global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
+__global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageArgs a) {
   static_assert(!CS || sizeof(T) == 4, "the compensated state is an FP32 form");
   using S = Sz<D, P>;
   using M = MfmaShape<D, P>;
+  constexpr int kBlock = MfmaLaunch<T, D, P>::KB;
   constexpr int NP = S::NP, NF = S::NF, NFP = S::NFP, NPERM = S::NPERM, NFF = M::NFF;
-  constexpr int NG = M::NG, NFK = M::NFK, KS = M::KS, MB = M::MB, TE = M::TE, RB = M::RB, NLO = M::NLO;
+  constexpr int NG = M::NG, NFK = M::NFK, KS = M::KS, MB = M::MB, RB = M::RB, NLO = M::NLO;
+  constexpr int TE = MfmaLaunch<T, D, P>::TE;
   constexpr int VW = 16 / sizeof(T);                        // values per 16-byte vector
   typedef T VT __attribute__((ext_vector_type(VW)));
   constexpr int PV = TE * NP / VW;                          // vectors per field plane of a tile
@@ -436,7 +447,9 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
         }
     }
     };
-    if (kALds) load_epi();
+    // wide workgroups need the registers during the MFMA chain: load just before the epilogue
+    constexpr bool kDefer = !kALds || MfmaLaunch<T, D, P>::WAVES > 4;
+    if (!kDefer) load_epi();
 #pragma unroll
     for (int t = 0; t < NI; ++t) cnc[t] = inc[t];
     const bool more = valid(it + bpx);
@@ -517,7 +530,8 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
         }
       }
     };
-#pragma unroll (kALds ? NG : 2)
+    constexpr int kNodeUnroll = kALds ? NG : 2;
+#pragma unroll kNodeUnroll
     for (int m = 0; m < NG; ++m) {
       const int n = 4 * m + g;
       T p = 0, vx = 0, vy = 0, vz = 0;
@@ -544,7 +558,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_mfma(StageArgs a) {
         if (i < NP) sR[q * TE * NP + el * NP + i] = acc[mb][j];
       }
     __syncthreads();
-    if (!kALds) load_epi();
+    if (kDefer) load_epi();
     // phase 4: time-integration epilogue on 16-byte vectors
     const T s6 = (T)(1.0 / 6.0), s3 = (T)(1.0 / 3.0), s2 = (T)0.5;
 #pragma unroll
@@ -685,10 +699,10 @@ void launch_lds(StageArgs& a, hipStream_t s) {
 
 template <typename T, int D, int P, bool LO, bool CS>
 void launch_mfma(StageArgs& a, hipStream_t s, int grid_pct) {
-  constexpr int TE = MfmaShape<D, P>::TE;
+  constexpr int TE = MfmaLaunch<T, D, P>::TE, KB = MfmaLaunch<T, D, P>::KB;
   static int occ = 0;
   if (!occ) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stage_mfma<T, D, P, LO, CS>, kBlock, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stage_mfma<T, D, P, LO, CS>, KB, 0) != hipSuccess ||
         occ < 1)
       occ = 1;
   }
@@ -697,7 +711,7 @@ void launch_mfma(StageArgs& a, hipStream_t s, int grid_pct) {
   const int64_t per_xcd_cap = (int64_t)(num_cus() / 8) * occ * std::max(grid_pct, 1) / 100;
   int bpx = (int)std::min<int64_t>(a.tiles_per_xcd, std::max<int64_t>(per_xcd_cap, 1));
   if (bpx < 1) bpx = 1;
-  hipLaunchKernelGGL((k_stage_mfma<T, D, P, LO, CS>), dim3(8 * bpx), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((k_stage_mfma<T, D, P, LO, CS>), dim3(8 * bpx), dim3(KB), 0, s, a);
 }
 
 template <typename T, int D, int P>

@@ -283,3 +283,21 @@ def test_room_full_size_fp32_vs_fp64_with_receivers():
     g64, g32 = out[dgk.FP64][0], out[dgk.FP32][0]
     assert rel(g32[0], g64[0]) <= 1e-5
     assert rel(out[dgk.FP32][1], out[dgk.FP64][1]) <= 1e-4
+
+
+@pytest.mark.parametrize("dim,p", [(2, 5), (3, 4), (3, 5)])
+def test_fp32_rk4_high_order_against_oracle(dim, p):
+    """FP32 form at the orders with their own workgroup shapes (3D P5: 8-wave workgroups)."""
+    m = mixed_mesh(dim, p, 4)
+    x = m.node_xyz
+    u0 = np.zeros((4, m.K * m.Np))
+    u0[0] = np.exp(-((x - x.mean(axis=0)) ** 2).sum(axis=1) / 0.05)
+    o = ob.Oracle(m, 340.0, 1.225, v0=(20.0, 10.0, 0.0), sign_mode=ob.SIGN_STABLE)
+    s = dgk.Solver(m, 340.0, 1.225, v0=(20.0, 10.0, 0.0), precision=dgk.FP32, flux_sign=dgk.SIGN_STABLE)
+    s.set_state(u0)
+    s.rk4(60, 1e-6)
+    ref = o.step(u0, 60, 1e-6)
+    g = s.get_state()
+    errs = [rel(g[q], ref[q]) for q in range(dim + 1)]
+    print(f"fp32 dim={dim} p={p}: {errs}")
+    assert max(errs) <= 1e-5
