@@ -32,12 +32,6 @@ int die(const std::string& s) {
   return EXIT_FAILURE;
 }
 
-// Receiver pressure by nearest element node (documented approximation of point evaluation).
-struct Receivers {
-  std::vector<int64_t> node;  // index into the p field
-  std::vector<std::vector<double>> series;
-};
-
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -137,18 +131,14 @@ int main(int argc, char** argv) {
     for (auto& s : cfg.sources) src.insert(src.end(), s.begin(), s.end());
     check(dgk_set_sources(ctx, (int)cfg.sources.size(), src.data(), top.node_xyz.data()));
   }
-  // receivers: nearest element node
-  Receivers rec;
-  for (auto& r : cfg.receivers) {
-    int64_t best = 0;
-    double bd = 1e300;
-    for (int64_t n = 0; n < KN; ++n) {
-      double d = 0;
-      for (int x = 0; x < 3; ++x) d += (top.node_xyz[3 * n + x] - r[x]) * (top.node_xyz[3 * n + x] - r[x]);
-      if (d < bd) { bd = d; best = n; }
-    }
-    rec.node.push_back(best);
-    rec.series.emplace_back();
+  // receivers: pressure interpolated at each point after every step, recorded on the device
+  const int nrec = (int)cfg.receivers.size();
+  long total_steps = 0;
+  for (double tt = cfg.timeStart; tt <= cfg.timeEnd; tt += cfg.timeStep) ++total_steps;  // the loop below
+  if (nrec) {
+    std::vector<double> rxyz;
+    for (auto& r : cfg.receivers) rxyz.insert(rxyz.end(), r.begin(), r.begin() + 3);
+    check(dgk_set_receivers(ctx, nrec, rxyz.data(), top.node_xyz.data(), (int)std::max(1L, total_steps)));
   }
   dgh::ViewWriter views;
   views.path = cfg.saveFile;
@@ -162,16 +152,8 @@ int main(int argc, char** argv) {
   std::vector<double> g_p(KN), g_rho(KN), g_v(3 * KN);
   auto flush = [&]() {
     if (pending.empty()) return;
-    if (rec.node.empty()) {
-      check(rk ? dgk_rk4(ctx, (int)pending.size(), cfg.timeStep, pending.data())
-               : dgk_euler(ctx, (int)pending.size(), cfg.timeStep, pending.data()));
-    } else {
-      for (size_t i = 0; i < pending.size(); ++i) {  // receivers sample the state after every step
-        check(rk ? dgk_rk4(ctx, 1, cfg.timeStep, &pending[i]) : dgk_euler(ctx, 1, cfg.timeStep, &pending[i]));
-        check(dgk_get_state(ctx, u.data()));
-        for (size_t r = 0; r < rec.node.size(); ++r) rec.series[r].push_back(u[rec.node[r]]);
-      }
-    }
+    check(rk ? dgk_rk4(ctx, (int)pending.size(), cfg.timeStep, pending.data())
+             : dgk_euler(ctx, (int)pending.size(), cfg.timeStep, pending.data()));
     pending.clear();
   };
   double t = cfg.timeStart, tDisplay = 0;
@@ -200,16 +182,20 @@ int main(int argc, char** argv) {
   flush();
   check(dgk_sync(ctx));
   if (dgk_check_finite(ctx) != DGK_OK) log("warning: the final state holds NaN or Inf");
-  if (!rec.node.empty()) {
+  if (nrec) {
+    int rows = 0;
+    check(dgk_get_receivers(ctx, nullptr, 0, &rows));
+    std::vector<double> series((size_t)rows * nrec);
+    check(dgk_get_receivers(ctx, series.data(), rows, &rows));
     std::string path = cfg.receiverFile.empty() ? cfg.saveFile + ".receivers.csv" : cfg.receiverFile;
     FILE* f = fopen(path.c_str(), "w");
     if (f) {
       fprintf(f, "step");
-      for (size_t r = 0; r < rec.node.size(); ++r) fprintf(f, ",p%zu", r);
+      for (int r = 0; r < nrec; ++r) fprintf(f, ",p%d", r);
       fprintf(f, "\n");
-      for (size_t i = 0; i < rec.series[0].size(); ++i) {
-        fprintf(f, "%zu", i + 1);
-        for (size_t r = 0; r < rec.node.size(); ++r) fprintf(f, ",%.17g", rec.series[r][i]);
+      for (int i = 0; i < rows; ++i) {
+        fprintf(f, "%d", i + 1);
+        for (int r = 0; r < nrec; ++r) fprintf(f, ",%.17g", series[(size_t)i * nrec + r]);
         fprintf(f, "\n");
       }
       fclose(f);
