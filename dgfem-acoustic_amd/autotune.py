@@ -67,7 +67,7 @@ def time_case(m, precision, variant, nsteps=10):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--dofs", type=float, default=4e6)
+    ap.add_argument("--dofs", type=float, default=8e6)
     ap.add_argument("--dims", default="1,2,3")
     ap.add_argument("--orders", default="1,2,3,4,5")
     ap.add_argument("--precisions", default="fp64,fp32")
