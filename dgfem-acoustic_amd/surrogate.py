@@ -178,6 +178,7 @@ class Surrogate:
         self.lo = np.asarray(lo, np.float64)
         self.hi = np.asarray(hi, np.float64)
         self.scale = float(scale)
+        self.cal = None  # scale of the disagreement that bounds the true error (calibrate())
         self.nets = [Net(out_dim=out_dim, seed=s, **net_kw) for s in seeds]
 
     def encode(self, src, rcv):
@@ -186,7 +187,9 @@ class Surrogate:
         rcv = np.atleast_2d(rcv)
         return np.concatenate([(src - self.lo) / (self.hi - self.lo), (rcv - self.lo) / (self.hi - self.lo)], axis=1)
 
-    def fit(self, src, rcv, series, steps, batch=256, lr=1e-3, seed=0):
+    def fit(self, src, rcv, series, steps, batch=256, lr=1e-3, seed=0, decay=(1.0, 0.3, 0.1)):
+        """Train every network on (src, rcv) -> series; the learning rate steps down through
+        `decay` over equal thirds of the run."""
         x = self.encode(src, rcv)
         t = np.asarray(series, np.float64) / self.scale
         # relative L2 loss per sample; pairs the pulse has not reached within the window carry
@@ -196,16 +199,32 @@ class Surrogate:
         out = []
         for k, n in enumerate(self.nets):
             n.set_data(x, t, w)
-            out.append(n.train(steps, batch, lr, seed + 101 * k))
+            parts = [steps // len(decay)] * len(decay)
+            parts[-1] += steps - sum(parts)
+            out.append(np.concatenate([n.train(ns, batch, lr * f, seed + 101 * k + 7 * j)
+                                       for j, (ns, f) in enumerate(zip(parts, decay)) if ns > 0]))
         return out
 
     def predict(self, src, rcv):
-        """(mean prediction, error estimate per query)."""
+        """(mean prediction, error estimate per query). With two networks whose errors are
+        independent and of equal size e, their difference has size sqrt(2) e and their mean's
+        error is e / sqrt(2): the estimate of the mean's relative error is half the relative
+        disagreement."""
         x = self.encode(src, rcv)
         ys = [n.forward(x).astype(np.float64) * self.scale for n in self.nets]
         mean = sum(ys) / len(ys)
-        est = rel_l2(ys[0], ys[1]) if len(ys) > 1 else np.zeros(len(mean))
-        return mean, est
+        dis = rel_l2(ys[0], ys[1]) if len(ys) > 1 else np.zeros(len(mean))
+        return mean, dis * (self.cal if self.cal is not None else 0.5)
+
+    def calibrate(self, src, rcv, series, quantile=0.9):
+        """Scale the disagreement so that on these held-out pairs (DG truths not used in
+        training) the estimate bounds the true relative error in `quantile` of the cases."""
+        self.cal = None
+        y, est = self.predict(src, rcv)
+        dis = 2.0 * est
+        err = rel_l2(y, np.asarray(series, np.float64))
+        self.cal = float(np.quantile(err / np.maximum(dis, 1e-30), quantile))
+        return self.cal
 
 
 class QueryService:

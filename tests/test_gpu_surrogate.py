@@ -92,20 +92,21 @@ def test_full_size_network_shape_and_query_time():
 
 
 def test_room_surrogate_with_dg_fallback():
-    """Learned part end to end on a small reverberant room (2 x 1.5 x 1 m, P3): DG impulse
-    responses from the batched solver with receivers, two networks trained on the GPU, the
-    disagreement gate and the DG fallback."""
+    """Learned part end to end on a small reverberant room (2 x 1.5 x 1 m, P3, 0.3 m pulse): DG
+    impulse responses from the batched solver with receivers, two networks trained on the GPU, a
+    disagreement gate calibrated on held-out DG pairs, and the DG fallback."""
+    import time
+
     import dgk
     import workloads as wl
 
-    m, ph, dt, sigma = wl.room_box(2.0, 1.5, 1.0, 0.125, 3)
+    m, ph, dt, _ = wl.room_box(2.0, 1.5, 1.0, 0.125, 3)
+    sigma = 0.3
     rng = np.random.default_rng(1000)
-    S, R, nsteps, stride = 32, 64, 2048, 4
+    S, R, nsteps, stride = 32, 128, 2048, 4
     lo, hi = np.full(3, 0.2), np.array([2.0, 1.5, 1.0]) - 0.2
     src = lo + (hi - lo) * rng.random((S, 3))
     rcv = lo + (hi - lo) * rng.random((S, R, 3))
-    import time
-
     t0 = time.perf_counter()
     series = surrogate.dg_impulse_responses(m, ph, dt, src, rcv, nsteps, sigma, stride)
     t_dg = (time.perf_counter() - t0) / S  # one DG solve answers one source's queries
@@ -120,11 +121,12 @@ def test_room_surrogate_with_dg_fallback():
     assert rel(series[0], single) <= 1e-4
     X_src, X_rcv, T = np.repeat(src, R, axis=0), rcv.reshape(-1, 3), series.reshape(S * R, -1)
     perm = rng.permutation(S * R)
-    tr, te = perm[:-256], perm[-256:]
+    te, ca, tr = perm[:256], perm[256:768], perm[768:]
     sur = surrogate.Surrogate(np.zeros(3), np.array([2.0, 1.5, 1.0]), out_dim=nsteps // stride,
                               scale=float(np.abs(T).max()), freq_sigma=0.1)
-    losses = sur.fit(X_src[tr], X_rcv[tr], T[tr], 2000, batch=256, lr=1e-3)
-    assert all(l[-50:].mean() < 0.5 * l[:10].mean() for l in losses)
+    losses = sur.fit(X_src[tr], X_rcv[tr], T[tr], 6000, batch=256, lr=2e-3)
+    assert all(l[-50:].mean() < 0.1 * l[:10].mean() for l in losses)
+    sur.calibrate(X_src[ca], X_rcv[ca], T[ca], quantile=0.9)
     y, est = sur.predict(X_src[te], X_rcv[te])
     err = surrogate.rel_l2(y, T[te])
     # the method without the network: answer with the nearest training pair (reciprocity-aware)
@@ -136,12 +138,15 @@ def test_room_surrogate_with_dg_fallback():
         nn_err.append(np.linalg.norm(T[tr][np.argmin(d)] - T[i]) / np.linalg.norm(T[i]))
     corr = float(np.corrcoef(est, err)[0, 1])
     t_q = sur.nets[0].forward_ms(256, 10) * 2 / 256 * 1e-3  # two networks per query
+    acc = est <= 0.05
     print(f"held-out rel L2: network median {np.median(err):.3f}, nearest-pair median {np.median(nn_err):.3f}; "
-          f"gate estimate vs true error corr {corr:.2f}; accepted at 2%: {(est <= 0.02).mean():.2f}; "
+          f"estimate vs true error corr {corr:.2f}; accepted at 5%: {acc.mean():.2f} "
+          f"(true error <= 5% in {(err[acc] <= 0.05).mean() if acc.any() else float('nan'):.2f} of them); "
           f"query {t_q * 1e6:.2f} us vs DG {t_dg * 1e3:.1f} ms per source")
-    assert np.median(err) < 0.8 * np.median(nn_err)
+    assert np.median(err) < 0.5 * np.median(nn_err)
     assert corr > 0.2
     assert t_dg / t_q >= 1000
+    assert acc.any() and (err[acc] <= 0.05).mean() >= 0.75  # the calibrated gate bounds what it accepts
     # fallback: declined queries are answered by DG exactly
     calls = []
 
@@ -150,9 +155,8 @@ def test_room_surrogate_with_dg_fallback():
         calls.append(i)
         return T[te][i]
 
-    svc = surrogate.QueryService(sur, dg_solve, tol=0.02)
+    svc = surrogate.QueryService(sur, dg_solve, tol=0.05)
     ans, by_net = svc.query(X_src[te], X_rcv[te])
     assert len(calls) == int((~by_net).sum())
     assert np.array_equal(ans[~by_net], T[te][~by_net])
-    if by_net.any():
-        assert np.median(err[by_net]) <= np.median(err)
+    assert np.array_equal(by_net, acc)
