@@ -119,3 +119,35 @@ def test_batch_rejects_bad_input():
     assert dgk.device().dgk_batch_get_state(b.h, 1, dgk._p(buf)) == dgk.DGK_ERR_INVALID  # no member 1
     with pytest.raises(dgk.DGKError):
         b.rk4(-1)
+
+
+def test_batch_high_order_members():
+    """Members at orders 4-5 (3D P5 FP32 runs 8-wave workgroups) in one batch: FP64 against the
+    oracle, FP32 against FP64."""
+    still = dict(c0=340.0, rho0=1.225, v0=(0.0, 0.0, 0.0))
+    ms = [dgk.gen_box(2, 2, 2, 1.0, 1.0, 1.0, 5, 0.1, 3, [1, 2, 1, 2, 1, 2], "1:Wall;2:Abs", conf(BCS)),
+          dgk.gen_box(3, 2, 2, 1.0, 0.8, 0.7, 4, 0.1, 4, [1, 1, 2, 2, 1, 2], "1:Wall;2:Abs", conf(BCS)),
+          dgk.gen_rect(5, 4, 1.0, 0.8, 5, [1, 2, 1, 2], "1:Wall;2:Abs", conf(BCS)),
+          dgk.gen_box(2, 3, 2, 0.9, 1.0, 0.6, 5, 0.05, 5, [2] * 6, "2:Abs", conf(BCS))]
+    dts = [1e-6, 1.5e-6, 2e-6, 1.2e-6]
+    u0 = []
+    for m in ms:
+        x = m.node_xyz
+        u = np.zeros((4, m.K * m.Np))
+        u[0] = np.exp(-((x - x.mean(axis=0)) ** 2).sum(axis=1) / 0.05)
+        u0.append(u)
+    out = {}
+    for prec in (dgk.FP64, dgk.FP32):
+        b = dgk.Batch(ms, [still] * len(ms), dts, precision=prec, flux_sign=dgk.SIGN_STABLE)
+        for i, u in enumerate(u0):
+            b.set_state(i, u)
+        b.rk4(40)
+        out[prec] = [b.get_state(i) for i in range(len(ms))]
+        b.close()
+    for i, (m, dt) in enumerate(zip(ms, dts)):
+        o = ob.Oracle(m, 340.0, 1.225, sign_mode=ob.SIGN_STABLE)
+        ref = o.step(u0[i], 40, dt)
+        e64 = rel(out[dgk.FP64][i], ref)
+        e32 = rel(out[dgk.FP32][i][0], out[dgk.FP64][i][0])
+        print(f"  member {i} dim={m.dim} p={m.order}: fp64 {e64:.2e}, fp32 p {e32:.2e}", flush=True)
+        assert e64 <= 1e-12 and e32 <= 1e-5
