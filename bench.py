@@ -17,7 +17,10 @@ Three legs, each timed over K RK4 steps after W warmup steps, with inputs reside
 A "step" is one RK4 step (4 stage launches) of every simulation in the leg. Counted unknowns
 are (dim+1) fields per node. Synthetic data: meshes and states are generated here.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--legs fp64,fp32,batch]
+* c1_fp64 -- the reference's CPU-runnable case (2D unit square, 3200 triangles, order 3, 1000
+  steps, FP64, reference flux sign), with the oracle's literal path timed on the host beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--legs fp64,fp32,batch,c1]
 """
 import argparse
 import json
@@ -203,6 +206,49 @@ def batch_leg(steps, warmup, members_checked=8, n=256):
                 sample=(meshes, phys, dts, pulses))
 
 
+def small_leg(no_cpu, steps=1000):
+    """The reference's CPU-runnable case: 2D unit square, 40x40 cells (3200 triangles), order 3,
+    reflecting walls, Gaussian pulse, 1000 RK4 steps of FP64 on the GPU (drop-in path, reference
+    flux sign), beside the oracle's literal path on the host over the first 50 steps."""
+    m, ph, dt, u0 = wl.square_2d()
+    dof = 3 * m.K * m.Np
+    s = dgk.Solver(m, ph["c0"], ph["rho0"])
+    s.set_state(u0)
+    s.rk4(25, dt)
+    s.set_state(u0)
+    s.sync()
+    t0 = time.perf_counter()
+    s.rk4(steps, dt)
+    s.sync()
+    wall = time.perf_counter() - t0
+    g = s.get_state()
+    s.close()
+    out = {"workload": "square2d-p3: unit square, 40x40 cells x 2 triangles (3200), order 3, reflecting walls, "
+                       "Gaussian pulse, FP64, reference flux sign", "steps": steps, "value": dof * 4 * steps / wall,
+           "ms_per_step": wall * 1e3 / steps, "unit": UNIT}
+    log(f"c1: {out['value']:.4g} DOF-stage/s ({wall * 1e3:.1f} ms for {steps} steps)")
+    if not no_cpu:
+        import oracle_binding as ob
+
+        threads = os.cpu_count() or 1
+        cpu_steps = 50  # bounded sample of the same run
+        o = ob.Oracle(m, ph["c0"], ph["rho0"], nthreads=threads)
+        t0 = time.perf_counter()
+        ref = o.step(u0, cpu_steps, dt, mode=ob.LITERAL)
+        sec = time.perf_counter() - t0
+        s = dgk.Solver(m, ph["c0"], ph["rho0"])
+        s.set_state(u0)
+        s.rk4(cpu_steps, dt)
+        g = s.get_state()
+        s.close()
+        out["cpu_literal"] = {"value": dof * 4 * cpu_steps / sec, "sample": f"first {cpu_steps} steps, {sec:.1f} s",
+                              "cores": threads, "kind": "port"}
+        out["rel_l2_vs_cpu_after_50_steps"] = float(np.linalg.norm(g - ref) / np.linalg.norm(ref))
+        log(f"c1: oracle literal {sec:.1f} s for {cpu_steps} steps on {threads} threads, GPU vs CPU rel L2 "
+            f"{out['rel_l2_vs_cpu_after_50_steps']:.2e}")
+    return out
+
+
 def cpu_baseline(meshes, phys, dts, pulses, members, steps=2):
     """Literal (reference loop structure) oracle on a bounded sample: `steps` RK4 steps of each member."""
     import oracle_binding as ob
@@ -236,7 +282,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="fp64,fp32,batch", help="comma list of fp64,fp32,batch")
+    ap.add_argument("--legs", default="fp64,fp32,batch,c1", help="comma list of fp64,fp32,batch,c1")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="first run a plain device copy of MB MiB (reference dispatch for HBM byte counters)")
     args = ap.parse_args()
@@ -265,6 +311,8 @@ def main():
             res[name] = single_leg(m, ph, dt, u0, prec, args.steps, args.warmup)
             log(f"{name} cube: {res[name]['rate']:.4g} DOF-stage/s, stage kernel "
                 f"{res[name]['roofline']['kernel_avg_us']} us ({res[name]['roofline']['achieved']} GB/s)")
+    if "c1" in legs:
+        res["c1"] = small_leg(args.no_cpu)
     if "batch" in legs:
         log(f"batch: {args.warmup} warmup + {args.steps} timed RK4 steps")
         res["batch"] = batch_leg(args.steps, args.warmup)
@@ -307,6 +355,8 @@ def main():
                               "workload": cube_cfg, "roofline": r["roofline"], "finite": r["finite"]}
         if "errs" in r:
             out["fp32_single"]["rel_l2_vs_fp64"] = {"after_steps": r["total_steps"], **r["errs"]}
+    if "c1" in res:
+        out["c1_fp64"] = res["c1"]
     out["cpu_baseline"] = None
     if not args.no_cpu and "batch" in res:
         log("cpu baseline: literal oracle on a sample of the ensemble, all host threads")
