@@ -22,7 +22,15 @@
  *                   matrices applied transposed, nested face region);
  *   ORC_FACTORIZED  the same discrete operator factorised for affine
  *                   elements (reference-element matrices x per-element scalars).
- * Parity status of each function is recorded in DESIGN.md.
+ * Pins (tests/test_oracle.py, tests/test_reference_element.py): reference matrices against
+ * exact rationals from Gmsh's node orderings (tests/golden/reference_matrices.json) and closed
+ * forms; literal = factorized to 1e-13; hashed = quadratic face search; one RK4 step = the
+ * degree-4 Taylor polynomial of the assembled operator, Euler = I + A; mass conservation,
+ * energy decay and steady constants with reflecting walls; the first-node BC rule.
+ * Parity unpinned: the reference's interior-penalty sign for 3D order >= 2, which follows
+ * Gmsh's (unavailable) quadrature point order -- ORC_SIGN_REFERENCE takes the first point of
+ * the collapsed Gauss-Legendre rule used here; Gmsh's own face-node order for order >= 4
+ * files written by Gmsh (the generators and reader here agree with each other).
  */
 #ifndef DGK_ORACLE_H
 #define DGK_ORACLE_H
