@@ -447,7 +447,8 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
         }
     }
     };
-    // wide workgroups need the registers during the MFMA chain: load just before the epilogue
+    // wide workgroups and operators read from global memory need the registers during the MFMA
+    // chain: their epilogue operands are loaded later (below)
     constexpr bool kDefer = !kALds || MfmaLaunch<T, D, P>::WAVES > 4;
     if (!kDefer) load_epi();
 #pragma unroll
@@ -544,6 +545,9 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
 #pragma unroll
       for (int r = 0; r < D; ++r) step(m * D + r, alpha[r] * p + beta[r][0] * vx + beta[r][1] * vy + beta[r][2] * vz);
     }
+    // 8-wave workgroups: the epilogue operands are issued here, in flight during the face k-steps
+    // (the volume coefficients are dead by now, which keeps the peak within 256 registers)
+    if (kDefer && kALds) load_epi();
 #pragma unroll
     for (int f = 0; f < NFK; ++f) {
       const int jk = 4 * f + g;
@@ -558,7 +562,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
         if (i < NP) sR[q * TE * NP + el * NP + i] = acc[mb][j];
       }
     __syncthreads();
-    if (kDefer) load_epi();
+    if (kDefer && !kALds) load_epi();
     // phase 4: time-integration epilogue on 16-byte vectors
     const T s6 = (T)(1.0 / 6.0), s3 = (T)(1.0 / 3.0), s2 = (T)0.5;
 #pragma unroll
