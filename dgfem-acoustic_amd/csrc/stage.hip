@@ -456,11 +456,18 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
     const bool more = valid(it + bpx);
     if (more) prefetch((xcd * tpx + it + bpx) * TE);
     __syncthreads();
-    // phase 2: nodal normal numerical fluxes scaled by sJ/|detJ| -> sB[q][e*RB + jk]
+    // phase 2: nodal normal numerical fluxes scaled by sJ/|detJ| -> sB[q][e*RB + jk]. All three face
+    // kinds share one branch-free form (no divergence between interior and boundary face nodes):
+    //   fl = A (F(qm) + F(qp)).n-with-speed-vne + H (qm - qp),   F(q).n = (vne p + rc2 v.n, vne v + p/rho n)
+    // interior    qp = neighbour trace, A = 1/2, vne = v0.n, H = s c0/2      (Rusanov)
+    // reflecting  qp = (p, v - 2 (v.n) n), A = 1/2, vne = v0.n, H = 0         (= n.F(p, v - (v.n) n))
+    // absorbing   qp = (0, v - (v.n) n),   A = 1/4, vne = 0,    H = c0/4
+    //             (= 1/4 [c0 p + rc2 v.n ; (p/rho + c0 v.n) n]; the qp part of F vanishes up to the
+    //             rounding of v.n - (v.n)(n.n))
 #pragma unroll
     for (int t = 0; t < NI; ++t) {
       const int idx = tid + t * kBlock;
-      if (idx >= TE * NFF) break;
+      if ((t + 1) * kBlock > TE * NFF && idx >= TE * NFF) continue;  // only the last round is partial
       const int e = idx / NFF, jk = idx - e * NFF, j = jk / NFP;
       const int ej = e * NF + j;
       const int2 nc = cnc[t];
@@ -470,25 +477,20 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
       const T qm[4] = {sU[0][on], sU[1][on], sU[2][on], sU[3][on]};
       const T vn = v0[0] * n[0] + v0[1] * n[1] + v0[2] * n[2];
       const T unm = qm[1] * n[0] + qm[2] * n[1] + qm[3] * n[2];
+      const bool inter = nc.x >= 0, refl = nc.x == -2;
+      const T gm = refl ? (T)2 : (T)1;
+      const T qp0 = inter ? iqp[t][0] : (refl ? qm[0] : (T)0);
+      T qpv[3];
+      for (int c = 0; c < 3; ++c) qpv[c] = inter ? iqp[t][1 + c] : qm[1 + c] - gm * unm * n[c];
+      const T unp = qpv[0] * n[0] + qpv[1] * n[1] + qpv[2] * n[2];
+      const T A = (inter || refl) ? (T)0.5 : (T)0.25;
+      const T H = inter ? (T)0.5 * ((nc.y >> 16) & 1 ? -c0 : c0) : (refl ? (T)0 : (T)0.25 * c0);
+      const T vne = (inter || refl) ? vn : (T)0;
       T fl[4];
-      if (nc.x >= 0) {
-        const T hc = (T)0.5 * ((nc.y >> 16) & 1 ? -c0 : c0);
-        const T qp[4] = {iqp[t][0], iqp[t][1], iqp[t][2], iqp[t][3]};
-        const T unp = qp[1] * n[0] + qp[2] * n[1] + qp[3] * n[2];
-        fl[0] = (T)0.5 * ((vn * qm[0] + rc2 * unm) + (vn * qp[0] + rc2 * unp)) + hc * (qm[0] - qp[0]);
-        for (int c = 0; c < 3; ++c)
-          fl[1 + c] = (T)0.5 * ((vn * qm[1 + c] + qm[0] * irho * n[c]) + (vn * qp[1 + c] + qp[0] * irho * n[c])) +
-                      hc * (qm[1 + c] - qp[1 + c]);
-      } else if (nc.x == -2) {
-        const T gv[3] = {qm[1] - unm * n[0], qm[2] - unm * n[1], qm[3] - unm * n[2]};
-        const T ung = gv[0] * n[0] + gv[1] * n[1] + gv[2] * n[2];
-        fl[0] = vn * qm[0] + rc2 * ung;
-        for (int c = 0; c < 3; ++c) fl[1 + c] = vn * gv[c] + qm[0] * irho * n[c];
-      } else {
-        fl[0] = (T)0.25 * (c0 * qm[0] + rc2 * unm);
-        const T tt = (T)0.25 * (qm[0] * irho + c0 * unm);
-        for (int c = 0; c < 3; ++c) fl[1 + c] = tt * n[c];
-      }
+      fl[0] = A * ((vne * qm[0] + rc2 * unm) + (vne * qp0 + rc2 * unp)) + H * (qm[0] - qp0);
+      for (int c = 0; c < 3; ++c)
+        fl[1 + c] = A * ((vne * qm[1 + c] + qm[0] * irho * n[c]) + (vne * qpv[c] + qp0 * irho * n[c])) +
+                    H * (qm[1 + c] - qpv[c]);
       for (int qq = 0; qq < 4; ++qq) sB[qq][e * RB + jk] = fs * fl[qq];
     }
     if (more) gather();  // next tile's neighbour traces, in flight during the MFMA phase
