@@ -223,6 +223,7 @@ void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* 
   // the thread-per-node kernel implements the plain FP32 form only
   const bool f32mfma = (o ? o->precision : DGK_FP64) == DGK_FP32 && c.variant == kVariantMfma;
   c.comp = f32mfma && !(flags & DGK_FP32_PLAIN_STATE);
+  c.aos = m->dim == 3 && c.variant == kVariantMfma;
   c.oplo = f32mfma && (flags & DGK_FP32_OPERATOR_CORRECTION);
   if (c.precision != DGK_FP64 && c.precision != DGK_FP32) throw InvalidArg("precision must be DGK_FP64 or DGK_FP32");
   if (c.sign_mode != DGK_SIGN_REFERENCE && c.sign_mode != DGK_SIGN_STABLE) throw InvalidArg("bad flux_sign");
@@ -660,26 +661,27 @@ double* staging(Ctx& c) {
 
 namespace {
 
-// Host u[4][K*Np] (FP64) <-> device planes with stride PS in the state precision.
+// Host u[4][K*Np] (FP64) <-> a device state buffer in the state precision and layout. FP64 field
+// planes are copied directly; otherwise through the FP64 staging planes and a conversion kernel.
 void upload_planes(Ctx& c, const double* u, void* dst) {
   const size_t row = (size_t)c.K * c.Np * sizeof(double);
-  if (c.precision == DGK_FP64) {
+  if (c.precision == DGK_FP64 && !c.aos) {
     DGK_HIP(hipMemcpy2DAsync(dst, c.PS * sizeof(double), u, row, row, 4, hipMemcpyHostToDevice, c.stream));
     return;
   }
   double* s = staging(c);
   DGK_HIP(hipMemcpy2DAsync(s, c.PS * sizeof(double), u, row, row, 4, hipMemcpyHostToDevice, c.stream));
-  convert_state(c, s, dst, 4 * c.PS, dst == c.U && c.comp ? c.Ulo : nullptr);
+  state_from_f64(c, s, dst, dst == c.U && c.comp ? c.Ulo : nullptr, 0, c.K * c.Np);
 }
 
 void download_planes(Ctx& c, const void* src, double* u) {
   const size_t row = (size_t)c.K * c.Np * sizeof(double);
-  if (c.precision == DGK_FP64) {
+  if (c.precision == DGK_FP64 && !c.aos) {
     DGK_HIP(hipMemcpy2DAsync(u, row, src, c.PS * sizeof(double), row, 4, hipMemcpyDeviceToHost, c.stream));
     return;
   }
   double* s = staging(c);
-  convert_state_back(c, src, s, 4 * c.PS, src == c.U && c.comp ? c.Ulo : nullptr);
+  state_to_f64(c, src, src == c.U && c.comp ? c.Ulo : nullptr, s, 0, c.K * c.Np);
   DGK_HIP(hipMemcpy2DAsync(u, row, s, c.PS * sizeof(double), row, 4, hipMemcpyDeviceToHost, c.stream));
 }
 
@@ -1168,21 +1170,21 @@ dgk_status batch_state_io(dgk_batch* b, int sim, double* u, bool to_device) {
       }
       c.nq = nq;
     }
-    double* s = f64 ? nullptr : dgk::staging(c);
+    // FP64 field planes are copied directly; otherwise through the FP64 staging planes
+    const bool direct = f64 && !c.aos;
+    double* s = direct ? nullptr : dgk::staging(c);
+    uint16_t* lo = c.comp ? c.Ulo : nullptr;
+    if (!to_device && !direct) dgk::state_to_f64(c, c.U, lo, s, off, n);
     for (int q = 0; q < 4; ++q) {
       const int64_t o = q * c.PS + off;
-      char* planes = (char*)c.U + o * c.tsize;
+      void* dev = direct ? (void*)((char*)c.U + o * c.tsize) : (void*)(s + o);
       double* host = u + q * n;
-      if (to_device) {
-        DGK_HIP(hipMemcpyAsync(f64 ? (void*)planes : (void*)(s + o), host, n * sizeof(double), hipMemcpyHostToDevice,
-                               c.stream));
-        if (!f64) dgk::convert_state(c, s + o, planes, n, c.comp ? c.Ulo + o : nullptr);
-      } else {
-        if (!f64) dgk::convert_state_back(c, planes, s + o, n, c.comp ? c.Ulo + o : nullptr);
-        DGK_HIP(hipMemcpyAsync(host, f64 ? (const void*)planes : (const void*)(s + o), n * sizeof(double),
-                               hipMemcpyDeviceToHost, c.stream));
-      }
+      if (to_device)
+        DGK_HIP(hipMemcpyAsync(dev, host, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+      else
+        DGK_HIP(hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     }
+    if (to_device && !direct) dgk::state_from_f64(c, s, c.U, lo, off, n);
     DGK_HIP(hipStreamSynchronize(c.stream));
   });
 }

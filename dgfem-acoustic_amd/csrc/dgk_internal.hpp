@@ -93,6 +93,10 @@ struct Ctx {
   uint16_t* Ulo = nullptr;  // FP32 compensated form: bf16 low word of U
   bool comp = false;        // FP32: compensated solution (U, Ulo)
   bool oplo = false;        // FP32: bf16 operator-residual correction on MFMA
+  // State layout of the device buffers U, A, V0, V1 (and Ulo): node-interleaved [Kpad][Np][4] for 3D
+  // contexts on the MFMA kernel, else four field planes [4][Kpad*Np] (plane stride PS). The host
+  // side always exchanges field planes.
+  bool aos = false;
   // Field planes that carry data. In 1D/2D the velocity components beyond dim stay identically
   // zero under the dynamics when they start at zero (no derivative or normal has those
   // components), so their planes are neither read nor written: nq = dim+1. A state with nonzero
@@ -152,10 +156,11 @@ void launch_stage(Ctx& c, StageArgs& a, hipStream_t s);
 void launch_source(Ctx& c, const Ctx::Src& s, double value, hipStream_t st);
 void launch_record(Ctx& c, hipStream_t st);  // no-op without receivers
 bool check_finite(Ctx& c);
-// FP64 -> state type; in the FP32 compensated form lo (if given) receives the bf16 low words
-void convert_state(Ctx& c, const double* src_dev, void* dst_dev, int64_t n, uint16_t* lo = nullptr);
-// state type -> FP64, adding the low words when lo is given
-void convert_state_back(Ctx& c, const void* src_dev, double* dst_dev, int64_t n, const uint16_t* lo = nullptr);
+// FP64 field planes on the device (plane stride PS; element nodes [off, off+n) of each plane) -> a
+// state buffer in the context's precision and layout; in the FP32 compensated form lo (if given)
+// receives the bf16 low words. And back, adding the low words when lo is given.
+void state_from_f64(Ctx& c, const double* s_dev, void* dst_dev, uint16_t* lo, int64_t off, int64_t n);
+void state_to_f64(Ctx& c, const void* src_dev, const uint16_t* lo, double* s_dev, int64_t off, int64_t n);
 const char* variant_name(int v);
 void mfma_shape(int dim, int p, int* KS, int* MB);
 // dgk_api.cpp

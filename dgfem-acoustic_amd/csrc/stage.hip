@@ -15,11 +15,13 @@
 // The face terms are exact at face nodes because every integrand is a polynomial of degree
 // <= 2p on straight faces.
 //
-// Data layout in HBM is the reference's state layout u[eq][element][node]; a workgroup
+// Data layout in HBM is the reference's state layout u[eq][element][node] in 1D/2D and the
+// node-interleaved u[element][node][eq] for 3D contexts on the MFMA kernel; a workgroup
 // stages a tile of TE consecutive elements in LDS (coalesced loads of TE*Np contiguous
-// values per equation), gathers neighbour traces through the device-built face table,
-// and walks tiles in an XCD-aware order (each XCD sweeps one contiguous slab of the mesh,
-// so neighbour traces are reused from that XCD's L2).
+// values per equation, or of one contiguous block), gathers neighbour traces through the
+// device-built face table, and walks tiles in an XCD-aware order (each XCD sweeps one
+// contiguous slab of the mesh, so neighbour traces are reused from that XCD's L2) or
+// round-robin.
 #include <algorithm>
 #include <type_traits>
 
@@ -296,13 +298,19 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   constexpr int TE = MfmaLaunch<T, D, P>::TE;
   constexpr int VW = 16 / sizeof(T);                        // values per 16-byte vector
   typedef T VT __attribute__((ext_vector_type(VW)));
+  typedef T V4 __attribute__((ext_vector_type(4)));         // the 4 fields of one node
   constexpr int PV = TE * NP / VW;                          // vectors per field plane of a tile
+  // 3D contexts keep the state node-interleaved ([element][node][field], 4 live fields): a tile is
+  // one contiguous block, a neighbour trace one 4-field load, a node's fields one LDS vector.
+  // 1D/2D keep field planes, so that their dead velocity planes cost no traffic.
+  constexpr bool AOS = D == 3;
+  constexpr int ES = AOS ? 4 : 1;  // state values per element node in a buffer's element stride
   constexpr int NE = (4 * PV + kBlock - 1) / kBlock;        // state vectors per thread
   constexpr int NI = (TE * NFF + kBlock - 1) / kBlock;      // face items per thread
   constexpr int NGEO = (TE * 9 + kBlock - 1) / kBlock;      // geometry values per thread
   constexpr int NFC = (TE * NF + kBlock - 1) / kBlock;      // element faces per thread
   typedef typename Mfma<T>::V V;
-  __shared__ __attribute__((aligned(16))) T sU[4][TE * NP];
+  __shared__ __attribute__((aligned(32))) T sU[4][TE * NP];
   __shared__ __attribute__((aligned(16))) T sB[4][TE * RB];  // face fluxes, then results as [4][TE*NP]
   __shared__ T sN[TE * NF][4];
   __shared__ T sG[TE][9];
@@ -333,7 +341,8 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   const T v0[3] = {(T)a.v0[0], (T)a.v0[1], (T)a.v0[2]};
   const int stage = a.stage;
   const int nq = a.nq;
-  for (int i = tid + nq * TE * NP; i < 4 * TE * NP; i += kBlock) (&sU[0][0])[i] = (T)0;  // planes that stay zero
+  if (!AOS)
+    for (int i = tid + nq * TE * NP; i < 4 * TE * NP; i += kBlock) (&sU[0][0])[i] = (T)0;  // planes that stay zero
   for (int i = tid; i < NF * NPERM * NFP; i += kBlock) sVmap[i] = a.vmap[i];
   for (int i = tid; i < NF * NFP; i += kBlock) sFmask[i] = a.fmask[i];
   if (kALds)
@@ -344,13 +353,15 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
 
   // Per-thread offsets that do not depend on the tile (every tile is full: the element count is
   // padded to a multiple of TE and padding elements carry zero state and geometry).
-  uint32_t voff[NE];  // plane*PS + r for the thread's state vectors (add e0*NP), in elements
-  int loff[NE];       // plane*TE*NP + r in LDS
+  // Field planes: plane*PS + r (add e0*NP) and plane*TE*NP + r. Node-interleaved: the tile is one
+  // block, v*VW in both (add e0*NP*4). In values.
+  uint32_t voff[NE];  // offset of the thread's state vectors in a state buffer
+  int loff[NE];       // and in the LDS tile
 #pragma unroll
   for (int t = 0; t < NE; ++t) {
     const int v = tid + t * kBlock, qq = v / PV, r = (v - qq * PV) * VW;
-    voff[t] = (uint32_t)(qq * PS + r);
-    loff[t] = qq * TE * NP + r;
+    voff[t] = AOS ? (uint32_t)(v * VW) : (uint32_t)(qq * PS + r);
+    loff[t] = AOS ? v * VW : qq * TE * NP + r;
   }
 
   // Software pipeline: while tile t is processed, tile t+1's state, geometry and face geometry,
@@ -363,7 +374,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   auto prefetch = [&](int e0) {
 #pragma unroll
     for (int t = 0; t < NE; ++t)
-      if (tid + t * kBlock < nq * PV) pst[t] = *byte_at((const VT*)Vin, (voff[t] + (uint32_t)(e0 * NP)) * (uint32_t)sizeof(T));
+      if (tid + t * kBlock < nq * PV) pst[t] = *byte_at((const VT*)Vin, (voff[t] + (uint32_t)(e0 * NP * ES)) * (uint32_t)sizeof(T));
 #pragma unroll
     for (int t = 0; t < NGEO; ++t) {
       const int idx = tid + t * kBlock;
@@ -390,9 +401,15 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
       const int k = ((tid + t * kBlock) % NFF) % NFP;
       if (inc[t].x >= 0) {
         const int pi = inc[t].x * NP + sVmap[(inc[t].y & 0xffff) * NFP + k];
+        if constexpr (AOS) {
+          const V4 w = *byte_at((const V4*)Vin, (uint32_t)pi * (uint32_t)sizeof(V4));
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
-          iqp[t][c] = c < nq ? *byte_at(Vin, (uint32_t)(c * PS + pi) * (uint32_t)sizeof(T)) : (T)0;
+          for (int c = 0; c < 4; ++c) iqp[t][c] = w[c];
+        } else {
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            iqp[t][c] = c < nq ? *byte_at(Vin, (uint32_t)(c * PS + pi) * (uint32_t)sizeof(T)) : (T)0;
+        }
       } else {
 #pragma unroll
         for (int c = 0; c < 4; ++c) iqp[t][c] = (T)0;
@@ -440,7 +457,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
 #pragma unroll
       for (int t = 0; t < NE; ++t)
         if (tid + t * kBlock < nq * PV) {
-          const uint32_t id = voff[t] + (uint32_t)(e0 * NP), ib = id * (uint32_t)sizeof(T);
+          const uint32_t id = voff[t] + (uint32_t)(e0 * NP * ES), ib = id * (uint32_t)sizeof(T);
           if (stage != 3 || CS) pu[t] = *byte_at((const VT*)U, ib);
           if (CS && stage == 3) plo[CS ? t : 0] = *byte_at((const uint2*)Ulo, id * 2u);
           pa[t] = *byte_at((const VT*)Acc, ib);
@@ -474,7 +491,13 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
       const T n[3] = {sN[ej][0], sN[ej][1], sN[ej][2]};
       const T fs = sN[ej][3];
       const int on = e * NP + sFmask[jk];
-      const T qm[4] = {sU[0][on], sU[1][on], sU[2][on], sU[3][on]};
+      T qm[4];
+      if constexpr (AOS) {
+        const V4 w = *(const V4*)(&sU[0][0] + on * 4);
+        for (int c = 0; c < 4; ++c) qm[c] = w[c];
+      } else {
+        for (int c = 0; c < 4; ++c) qm[c] = sU[c][on];
+      }
       const T vn = v0[0] * n[0] + v0[1] * n[1] + v0[2] * n[2];
       const T unm = qm[1] * n[0] + qm[2] * n[1] + qm[3] * n[2];
       const bool inter = nc.x >= 0, refl = nc.x == -2;
@@ -539,10 +562,18 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
       const int n = 4 * m + g;
       T p = 0, vx = 0, vy = 0, vz = 0;
       if (n < NP) {
-        p = sU[0][el * NP + n];
-        vx = sU[1][el * NP + n];
-        vy = sU[2][el * NP + n];
-        vz = sU[3][el * NP + n];
+        if constexpr (AOS) {
+          const V4 w = *(const V4*)(&sU[0][0] + (el * NP + n) * 4);
+          p = w[0];
+          vx = w[1];
+          vy = w[2];
+          vz = w[3];
+        } else {
+          p = sU[0][el * NP + n];
+          vx = sU[1][el * NP + n];
+          vy = sU[2][el * NP + n];
+          vz = sU[3][el * NP + n];
+        }
       }
 #pragma unroll
       for (int r = 0; r < D; ++r) step(m * D + r, alpha[r] * p + beta[r][0] * vx + beta[r][1] * vy + beta[r][2] * vz);
@@ -555,13 +586,13 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
       const int jk = 4 * f + g;
       step(D * NG + f, jk < NFF ? sB[q][el * RB + jk] : (T)0);
     }
-    __syncthreads();  // face fluxes consumed; sB now receives the results as [4][TE*NP]
+    __syncthreads();  // face fluxes consumed; sB now receives the results in the LDS tile's layout
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int i = 16 * mb + d_row<T>(lane, j);
-        if (i < NP) sR[q * TE * NP + el * NP + i] = acc[mb][j];
+        if (i < NP) sR[AOS ? (el * NP + i) * 4 + q : q * TE * NP + el * NP + i] = acc[mb][j];
       }
     __syncthreads();
     if (kDefer && !kALds) load_epi();
@@ -571,7 +602,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
     for (int t = 0; t < NE; ++t) {
       if (tid + t * kBlock >= nq * PV) break;
       const VT kq = dt * *(const VT*)(sR + loff[t]);
-      const uint32_t id = voff[t] + (uint32_t)(e0 * NP), ib = id * (uint32_t)sizeof(T);
+      const uint32_t id = voff[t] + (uint32_t)(e0 * NP * ES), ib = id * (uint32_t)sizeof(T);
       VT* const pAcc = byte_at((VT*)Acc, ib);
       VT* const pOut = byte_at((VT*)Vout, ib);
       VT* const pU = byte_at((VT*)U, ib);
@@ -617,13 +648,15 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   }
 }
 
+// idx holds pressure-plane offsets e*Np + node; es = 4 for node-interleaved state, else 1
 template <typename T>
 __global__ void k_source(T* __restrict__ U, uint16_t* __restrict__ Ulo, const int* __restrict__ idx, int64_t n,
-                         T value) {
+                         T value, int es) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) {
-    U[idx[i]] = value;
-    if (Ulo) Ulo[idx[i]] = 0;
+    const int64_t j = (int64_t)idx[i] * es;
+    U[j] = value;
+    if (Ulo) Ulo[j] = 0;
   }
 }
 
@@ -633,7 +666,7 @@ __global__ void k_source(T* __restrict__ U, uint16_t* __restrict__ Ulo, const in
 template <typename T>
 __global__ void k_record(const T* __restrict__ U, const uint16_t* __restrict__ Ulo, const int* __restrict__ base,
                          const T* __restrict__ w, int n, int Np, T* __restrict__ out, int* __restrict__ counter,
-                         int cap) {
+                         int cap, int es) {
   __shared__ int row;
   if (threadIdx.x == 0) row = *counter;
   __syncthreads();
@@ -643,8 +676,9 @@ __global__ void k_record(const T* __restrict__ U, const uint16_t* __restrict__ U
       const int b = base[r];
       T acc = 0;
       for (int k = 0; k < Np; ++k) {
-        T u = U[b + k];
-        if (Ulo) u += (T)bf16_bits_to_f32(Ulo[b + k]);
+        const int64_t j = (int64_t)(b + k) * es;  // pressure of node k
+        T u = U[j];
+        if (Ulo) u += (T)bf16_bits_to_f32(Ulo[j]);
         acc += w[r * Np + k] * u;
       }
       out[(int64_t)r0 * n + r] = acc;
@@ -662,17 +696,33 @@ __global__ void k_finite(const T* __restrict__ x, int64_t n, int* __restrict__ f
   if (__any(bad) && (threadIdx.x % 64) == 0) atomicOr(flag, 1);
 }
 
-// FP64 -> FP32 (+ bf16 low word of the remainder when lo is given) and back.
-__global__ void k_d2f(const double* __restrict__ s, float* __restrict__ d, uint16_t* __restrict__ lo, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const float h = (float)s[i];
-    d[i] = h;
-    if (lo) lo[i] = (uint16_t)f32_to_bf16_bits((float)(s[i] - (double)h));
+// FP64 field planes (plane stride PS; element nodes [off, off+n) of each plane) -> a state buffer in
+// the context's precision and layout (es = 4: node-interleaved, else planes), with the bf16 low word
+// of the FP32 remainder when lo is given; and back.
+template <typename T>
+__global__ void k_from_f64(const double* __restrict__ s, int64_t PS, int64_t off, int64_t n, T* __restrict__ d,
+                           uint16_t* __restrict__ lo, int es) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 4 * n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = es == 4 ? (i & 3) : i / n, node = off + (es == 4 ? i >> 2 : i - q * n);
+    const int64_t di = es == 4 ? node * 4 + q : q * PS + node;
+    const double x = s[q * PS + node];
+    if constexpr (sizeof(T) == 8) {
+      d[di] = x;
+    } else {
+      const float h = (float)x;
+      d[di] = h;
+      if (lo) lo[di] = (uint16_t)f32_to_bf16_bits((float)(x - (double)h));
+    }
   }
 }
-__global__ void k_f2d(const float* __restrict__ s, const uint16_t* __restrict__ lo, double* __restrict__ d, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    d[i] = (double)s[i] + (lo ? (double)bf16_bits_to_f32(lo[i]) : 0.0);
+template <typename T>
+__global__ void k_to_f64(const T* __restrict__ d, const uint16_t* __restrict__ lo, int64_t PS, int64_t off, int64_t n,
+                         double* __restrict__ s, int es) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 4 * n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = es == 4 ? (i & 3) : i / n, node = off + (es == 4 ? i >> 2 : i - q * n);
+    const int64_t di = es == 4 ? node * 4 + q : q * PS + node;
+    s[q * PS + node] = (double)d[di] + (lo ? (double)bf16_bits_to_f32(lo[di]) : 0.0);
+  }
 }
 
 int num_cus() {
@@ -755,10 +805,10 @@ void launch_source(Ctx& c, const Ctx::Src& src, double value, hipStream_t st) {
   unsigned nb = (unsigned)((src.n + 255) / 256);
   if (c.precision == DGK_FP64)
     hipLaunchKernelGGL(k_source<double>, dim3(nb), dim3(256), 0, st, (double*)c.U, (uint16_t*)nullptr, src.idx,
-                       src.n, value);
+                       src.n, value, c.aos ? 4 : 1);
   else
     hipLaunchKernelGGL(k_source<float>, dim3(nb), dim3(256), 0, st, (float*)c.U, (uint16_t*)(c.comp ? c.Ulo : nullptr),
-                       src.idx, src.n, (float)value);
+                       src.idx, src.n, (float)value, c.aos ? 4 : 1);
   DGK_HIP(hipGetLastError());
 }
 
@@ -767,11 +817,11 @@ void launch_record(Ctx& c, hipStream_t st) {
   if (c.precision == DGK_FP64)
     hipLaunchKernelGGL(k_record<double>, dim3(1), dim3(256), 0, st, (const double*)c.U, (const uint16_t*)nullptr,
                        c.rec.base, (const double*)c.rec.w, c.rec.n, c.Np, (double*)c.rec.out, c.rec.counter,
-                       c.rec.cap);
+                       c.rec.cap, c.aos ? 4 : 1);
   else
     hipLaunchKernelGGL(k_record<float>, dim3(1), dim3(256), 0, st, (const float*)c.U,
                        (const uint16_t*)(c.comp ? c.Ulo : nullptr), c.rec.base, (const float*)c.rec.w, c.rec.n,
-                       c.Np, (float*)c.rec.out, c.rec.counter, c.rec.cap);
+                       c.Np, (float*)c.rec.out, c.rec.counter, c.rec.cap, c.aos ? 4 : 1);
   DGK_HIP(hipGetLastError());
 }
 
@@ -789,21 +839,24 @@ bool check_finite(Ctx& c) {
   return h == 0;
 }
 
-void convert_state(Ctx& c, const double* src, void* dst, int64_t n, uint16_t* lo) {
-  if (c.precision == DGK_FP64) {
-    DGK_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-    return;
-  }
-  hipLaunchKernelGGL(k_d2f, dim3(1024), dim3(256), 0, c.stream, src, (float*)dst, lo, n);
+void state_from_f64(Ctx& c, const double* s, void* dst, uint16_t* lo, int64_t off, int64_t n) {
+  const int es = c.aos ? 4 : 1;
+  if (c.precision == DGK_FP64)
+    hipLaunchKernelGGL(k_from_f64<double>, dim3(1024), dim3(256), 0, c.stream, s, c.PS, off, n, (double*)dst,
+                       (uint16_t*)nullptr, es);
+  else
+    hipLaunchKernelGGL(k_from_f64<float>, dim3(1024), dim3(256), 0, c.stream, s, c.PS, off, n, (float*)dst, lo, es);
   DGK_HIP(hipGetLastError());
 }
 
-void convert_state_back(Ctx& c, const void* src, double* dst, int64_t n, const uint16_t* lo) {
-  if (c.precision == DGK_FP64) {
-    DGK_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-    return;
-  }
-  hipLaunchKernelGGL(k_f2d, dim3(1024), dim3(256), 0, c.stream, (const float*)src, lo, dst, n);
+void state_to_f64(Ctx& c, const void* src, const uint16_t* lo, double* s, int64_t off, int64_t n) {
+  const int es = c.aos ? 4 : 1;
+  if (c.precision == DGK_FP64)
+    hipLaunchKernelGGL(k_to_f64<double>, dim3(1024), dim3(256), 0, c.stream, (const double*)src,
+                       (const uint16_t*)nullptr, c.PS, off, n, s, es);
+  else
+    hipLaunchKernelGGL(k_to_f64<float>, dim3(1024), dim3(256), 0, c.stream, (const float*)src, lo, c.PS, off, n, s,
+                       es);
   DGK_HIP(hipGetLastError());
 }
 

@@ -109,13 +109,15 @@ def test_cube_3d_small_both_signs_fp64():
         assert rel(s.get_state(), o.step(u0, 200, dt)) <= 1e-12
 
 
-def test_euler_and_sources_fp64():
-    m = mixed_mesh(2, 2, 8)
+@pytest.mark.parametrize("dim", [2, 3])
+def test_euler_and_sources_fp64(dim):
+    m = mixed_mesh(dim, 2, 8 if dim == 2 else 4)
     o = ob.Oracle(m, 340.0, 1.225, v0=(10.0, 0.0, 0.0))
     s = dgk.Solver(m, 340.0, 1.225, v0=(10.0, 0.0, 0.0))
-    src = np.array([[0, 0.5, 0.4, 0, 0.2, 1.0, 800.0, 0.3, 1e-4],
-                    [1, 0.3, 0.4, 0, 0.1, 2.0, 500.0, 0.0, 1.0],
-                    [1, 0.5, 0.4, 0, 0.1, 2.0, 500.0, np.pi, 1.0]])
+    z = 0.3 if dim == 3 else 0.0
+    src = np.array([[0, 0.5, 0.4, z, 0.2, 1.0, 800.0, 0.3, 1e-4],
+                    [1, 0.3, 0.4, z, 0.1, 2.0, 500.0, 0.0, 1.0],
+                    [1, 0.5, 0.4, z, 0.1, 2.0, 500.0, np.pi, 1.0]])
     dt = 1e-6
     ts = np.cumsum(np.full(60, dt)) - dt
     u0 = np.zeros((4, m.K * m.Np))
@@ -221,12 +223,14 @@ def test_fp32_compensated_state_round_trip():
     assert 2.0 ** -27 <= rel(p.get_state(), u) <= 2.0 ** -23
 
 
-def test_fp32_euler_and_sources_compensated():
-    m = mixed_mesh(2, 3, 8)
+@pytest.mark.parametrize("dim", [2, 3])
+def test_fp32_euler_and_sources_compensated(dim):
+    m = mixed_mesh(dim, 3, 8 if dim == 2 else 4)
     o = ob.Oracle(m, 340.0, 1.225, sign_mode=ob.SIGN_STABLE)
     s = dgk.Solver(m, 340.0, 1.225, precision=dgk.FP32, flux_sign=dgk.SIGN_STABLE)
-    u0 = wl.gaussian(m, (0.5, 0.4, 0.0), 0.01)
-    src = np.array([[0, 0.3, 0.3, 0, 0.1, 0.5, 2000.0, 0.2, 4e-5]])
+    z = 0.3 if dim == 3 else 0.0
+    u0 = wl.gaussian(m, (0.5, 0.4, z), 0.01)
+    src = np.array([[0, 0.3, 0.3, z, 0.1, 0.5, 2000.0, 0.2, 4e-5]])
     dt = 1e-6
     ts = np.arange(60) * dt
     s.set_sources(src)
@@ -237,7 +241,7 @@ def test_fp32_euler_and_sources_compensated():
     ref = o.step(u0, 60, dt, t_steps=ts, sources=src)
     ref = o.step(ref, 10, dt, t_steps=ts[:10] + 60 * dt, sources=src, euler=True)
     ref = o.step(ref, 30, dt, t_steps=ts[:30] + 70 * dt, sources=src)
-    assert rel(s.get_state()[:3], ref[:3]) <= 1e-5
+    assert rel(s.get_state()[:dim + 1], ref[:dim + 1]) <= 1e-5
 
 
 @pytest.mark.parametrize("dim", [1, 2])
