@@ -107,26 +107,31 @@ void upload_ops(Ctx& c, const RefElem& R, const std::vector<int>& vmap) {
   c.ops.vmap = (int*)c.alloc(vmap.size() * sizeof(int));
   DGK_HIP(hipMemcpy(c.ops.vmap, vmap.data(), vmap.size() * sizeof(int), hipMemcpyHostToDevice));
   // MFMA A fragments: k-step s < d*NG covers W_r (r = s % d) at node 4*(s/d) + g for lane group g;
-  // later k-steps cover face entry 4*(s - d*NG) + g of -LIFT. Rows beyond Np are zero.
-  int KS = 0, MB = 0;
-  mfma_shape(R.dim, R.p, &KS, &MB);
-  const int Np = R.Np, NFF = R.Nf * R.Nfp, NG = (Np + 3) / 4, d = R.dim;
-  std::vector<T> A((size_t)MB * KS * 64, (T)0);
-  for (int mb = 0; mb < MB; ++mb)
+  // later k-steps cover face entry 4*(s - d*NG) + g of -LIFT. Rows beyond Np are zero. Layout
+  // [block][k-step][lane]: MBF 16-row blocks (lane l: row 16*mb + l%16, lane group l/16), then NT
+  // 4-row tail pieces for the 4x4x1 MFMA (lane l: row 16*MBF + 4t + l%4, lane group l/16).
+  int KS = 0, MB = 0, NT = 0;
+  mfma_shape(R.dim, R.p, sizeof(T) == 4, c.oplo, &KS, &MB, &NT);
+  const int Np = R.Np, NFF = R.Nf * R.Nfp, NG = (Np + 3) / 4, d = R.dim, MBF = NT ? MB - 1 : MB;
+  auto coef = [&](int i, int s, int g) {  // operator entry of row i at k-step s, lane group g
+    double v = 0;
+    if (i < Np) {
+      if (s < d * NG) {
+        int r = s % d, n = 4 * (s / d) + g;
+        if (n < Np) v = R.W[(r * Np + i) * Np + n];
+      } else {
+        int jk = 4 * (s - d * NG) + g;
+        if (jk < NFF) v = -R.LIFT[i * NFF + jk];
+      }
+    }
+    return v;
+  };
+  std::vector<T> A((size_t)(MBF + NT) * KS * 64, (T)0);
+  for (int mb = 0; mb < MBF + NT; ++mb)
     for (int s = 0; s < KS; ++s)
       for (int l = 0; l < 64; ++l) {
-        int i = 16 * mb + (l % 16), g = l / 16;
-        double v = 0;
-        if (i < Np) {
-          if (s < d * NG) {
-            int r = s % d, n = 4 * (s / d) + g;
-            if (n < Np) v = R.W[(r * Np + i) * Np + n];
-          } else {
-            int jk = 4 * (s - d * NG) + g;
-            if (jk < NFF) v = -R.LIFT[i * NFF + jk];
-          }
-        }
-        A[((size_t)mb * KS + s) * 64 + l] = (T)v;
+        const int i = mb < MBF ? 16 * mb + (l % 16) : 16 * MBF + 4 * (mb - MBF) + (l % 4);
+        A[((size_t)mb * KS + s) * 64 + l] = (T)coef(i, s, l / 16);
       }
   c.ops.Amf = c.alloc(A.size() * sizeof(T));
   DGK_HIP(hipMemcpy(c.ops.Amf, A.data(), A.size() * sizeof(T), hipMemcpyHostToDevice));
@@ -143,17 +148,7 @@ void upload_ops(Ctx& c, const RefElem& R, const std::vector<int>& vmap) {
     for (int mb = 0; mb < MB; ++mb)
       for (int s = 0; s < KS; ++s)
         for (int l = 0; l < 64; ++l) {
-          int i = 16 * mb + (l % 16), g = l / 16;
-          double v = 0;
-          if (i < Np) {
-            if (s < d * NG) {
-              int r = s % d, n = 4 * (s / d) + g;
-              if (n < Np) v = R.W[(r * Np + i) * Np + n];
-            } else {
-              int jk = 4 * (s - d * NG) + g;
-              if (jk < NFF) v = -R.LIFT[i * NFF + jk];
-            }
-          }
+          const double v = coef(16 * mb + (l % 16), s, l / 16);
           float hi = (float)v;
           lo[(((size_t)mb * NLO + s / 8) * 64 + l) * 8 + s % 8] = to_bf16((float)(v - (double)hi));
         }

@@ -162,7 +162,8 @@ bool check_finite(Ctx& c);
 void state_from_f64(Ctx& c, const double* s_dev, void* dst_dev, uint16_t* lo, int64_t off, int64_t n);
 void state_to_f64(Ctx& c, const void* src_dev, const uint16_t* lo, double* s_dev, int64_t off, int64_t n);
 const char* variant_name(int v);
-void mfma_shape(int dim, int p, int* KS, int* MB);
+// MFMA k-steps, 16-row blocks and 4-row tail pieces (0: none) of the stage kernel's operator
+void mfma_shape(int dim, int p, bool fp32, bool oplo, int* KS, int* MB, int* NT);
 // dgk_api.cpp
 extern thread_local std::string g_err;
 void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* o);

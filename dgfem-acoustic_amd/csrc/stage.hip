@@ -228,6 +228,16 @@ __device__ __forceinline__ Mfma<float>::V mma_bf16(bf16x8 a, bf16x8 b, Mfma<floa
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
+// 16 blocks of 4x4x1 into an FP32 accumulator; never called for other accumulator types
+template <typename VV, typename TT>
+__device__ __forceinline__ VV mma_4x4x1(TT, TT, VV c) {
+  return c;
+}
+template <>
+__device__ __forceinline__ Mfma<float>::V mma_4x4x1(float a, float b, Mfma<float>::V c) {
+  return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0);
+}
+
 template <int D, int P>
 struct MfmaShape {
   using S = Sz<D, P>;
@@ -238,6 +248,17 @@ struct MfmaShape {
   static constexpr int MB = (NP + 15) / 16;   // 16-row blocks
   static constexpr int RB = NFF > NP ? NFF : NP;
   static constexpr int NLO = (KS + 7) / 8;    // bf16 low-order correction MFMAs per m-block
+  // The last 16-row block holds only RT = Np - 16(MB-1) operator rows. In the FP32 form without
+  // the operator correction, when they fit one 4-row piece (3D orders 1, 3, 4; 2D order 1; 1D up to
+  // order 3), that piece runs on v_mfma_f32_4x4x1_16b_f32 (16 blocks of 4x4, 8 cycles) instead of
+  // one 16x16x4 MFMA (32 cycles) per k-step (two or three pieces measured slower: one more operator
+  // read from LDS per k-step, and at 3D order 5 register spills): block b = 4*kc + cg takes the k-step's lane group kc and columns 4cg..4cg+3, which is
+  // exactly the B operand the lane already holds (lane 16kc + 4cg + j); the four k-residue partial
+  // sums are added across lane groups after the chain. Lane layout (in-bounds probe on gfx950): A
+  // lane 4b + i holds row i of block b, B lane 4b + j column j, accumulator register i of lane
+  // 4b + j is D_b[i][j].
+  static constexpr int RT = NP - 16 * (MB - 1), NT4 = (RT + 3) / 4;
+  static constexpr int tail(bool fp32, bool lo) { return fp32 && !lo && NT4 == 1 ? 1 : 0; }
 };
 
 // Output mapping of the 16x16x4 accumulator: row of register j in lane l (column is l % 16).
@@ -295,6 +316,9 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   constexpr int kBlock = MfmaLaunch<T, D, P>::KB;
   constexpr int NP = S::NP, NF = S::NF, NFP = S::NFP, NPERM = S::NPERM, NFF = M::NFF;
   constexpr int NG = M::NG, NFK = M::NFK, KS = M::KS, MB = M::MB, RB = M::RB, NLO = M::NLO;
+  constexpr int NT4 = M::tail(sizeof(T) == 4, LO);  // 4-row tail pieces (0: all blocks 16x16x4)
+  constexpr int MBF = NT4 ? MB - 1 : MB;             // 16x16x4 row blocks
+  constexpr int NFRAG = MBF + NT4;                   // A fragments per k-step
   constexpr int TE = MfmaLaunch<T, D, P>::TE;
   constexpr int VW = 16 / sizeof(T);                        // values per 16-byte vector
   typedef T VT __attribute__((ext_vector_type(VW)));
@@ -320,8 +344,8 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   // of a gfx950 CU (fewer resident workgroups, but no per-k-step global loads).
   constexpr size_t kLdsRest = sizeof(T) * (4 * TE * NP + 4 * TE * RB + TE * NF * 4 + TE * 9) +
                               sizeof(int) * (NF * NPERM * NFP + NF * NFP) + (LO ? 16 * MB * NLO * 64 : 0);
-  constexpr bool kALds = MB * KS * 64 * sizeof(T) + kLdsRest <= 160 * 1024 - 512;
-  __shared__ T sA[kALds ? MB * KS * 64 : 1];
+  constexpr bool kALds = NFRAG * KS * 64 * sizeof(T) + kLdsRest <= 160 * 1024 - 512;
+  __shared__ T sA[kALds ? NFRAG * KS * 64 : 1];
   __shared__ bf16x8 sAlo[LO ? MB * NLO * 64 : 1];
   T* const sR = &sB[0][0];
 
@@ -346,7 +370,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   for (int i = tid; i < NF * NPERM * NFP; i += kBlock) sVmap[i] = a.vmap[i];
   for (int i = tid; i < NF * NFP; i += kBlock) sFmask[i] = a.fmask[i];
   if (kALds)
-    for (int i = tid; i < MB * KS * 64; i += kBlock) sA[i] = Af[i];
+    for (int i = tid; i < NFRAG * KS * 64; i += kBlock) sA[i] = Af[i];
   const T* __restrict__ pA = (kALds ? (const T*)sA : Af) + lane;
   if (LO)
     for (int i = tid; i < MB * NLO * 64; i += kBlock) sAlo[i] = ((const bf16x8*)a.Alo)[i];
@@ -533,9 +557,11 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
         }
       }
     }
-    V acc[MB];
+    V acc[MBF ? MBF : 1], acct[NT4 ? NT4 : 1];
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb) acc[mb] = V{0, 0, 0, 0};
+    for (int mb = 0; mb < MBF; ++mb) acc[mb] = V{0, 0, 0, 0};
+#pragma unroll
+    for (int t = 0; t < NT4; ++t) acct[t] = V{0, 0, 0, 0};
     bf16x8 blo;
     // Fragments read from global memory (shapes whose operator does not fit in LDS) are reloaded per
     // tile through an opaque pointer and the node loop is only partly unrolled: otherwise the
@@ -544,7 +570,9 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
     if (!kALds) asm volatile("" : "+v"(pAt));
     auto step = [&](const int s, const T b) {
 #pragma unroll
-      for (int mb = 0; mb < MB; ++mb) acc[mb] = Mfma<T>::mma(pAt[(mb * KS + s) * 64], b, acc[mb]);
+      for (int mb = 0; mb < MBF; ++mb) acc[mb] = Mfma<T>::mma(pAt[(mb * KS + s) * 64], b, acc[mb]);
+#pragma unroll
+      for (int t = 0; t < NT4; ++t) acct[t] = mma_4x4x1(pAt[((MBF + t) * KS + s) * 64], b, acct[t]);
       if constexpr (LO) {
         blo[s % 8] = (__bf16)b;
         if (s % 8 == 7 || s == KS - 1) {
@@ -588,11 +616,22 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
     }
     __syncthreads();  // face fluxes consumed; sB now receives the results in the LDS tile's layout
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb)
+    for (int mb = 0; mb < MBF; ++mb)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int i = 16 * mb + d_row<T>(lane, j);
         if (i < NP) sR[AOS ? (el * NP + i) * 4 + q : q * TE * NP + el * NP + i] = acc[mb][j];
+      }
+#pragma unroll
+    for (int t = 0; t < NT4; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        // lane group g holds the partial sum over k-steps' lane group g: add the four
+        T v = acct[t][j];
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        const int i = 16 * MBF + 4 * t + j;
+        if (g == 0 && i < NP) sR[AOS ? (el * NP + i) * 4 + q : q * TE * NP + el * NP + i] = v;
       }
     __syncthreads();
     if (kDefer && !kALds) load_epi();
@@ -868,14 +907,19 @@ const char* variant_name(int v) {
   }
 }
 
-void mfma_shape(int dim, int p, int* KS, int* MB) {
-#define DGK_SHAPE(D, P) \
-  if (dim == D && p == P) { *KS = MfmaShape<D, P>::KS; *MB = MfmaShape<D, P>::MB; return; }
+void mfma_shape(int dim, int p, bool fp32, bool oplo, int* KS, int* MB, int* NT) {
+#define DGK_SHAPE(D, P)                                                                   \
+  if (dim == D && p == P) {                                                               \
+    *KS = MfmaShape<D, P>::KS;                                                            \
+    *MB = MfmaShape<D, P>::MB;                                                            \
+    *NT = MfmaShape<D, P>::tail(fp32, oplo);                                              \
+    return;                                                                               \
+  }
   DGK_SHAPE(1, 1) DGK_SHAPE(1, 2) DGK_SHAPE(1, 3) DGK_SHAPE(1, 4) DGK_SHAPE(1, 5)
   DGK_SHAPE(2, 1) DGK_SHAPE(2, 2) DGK_SHAPE(2, 3) DGK_SHAPE(2, 4) DGK_SHAPE(2, 5)
   DGK_SHAPE(3, 1) DGK_SHAPE(3, 2) DGK_SHAPE(3, 3) DGK_SHAPE(3, 4) DGK_SHAPE(3, 5)
 #undef DGK_SHAPE
-  *KS = 0; *MB = 0;
+  *KS = 0; *MB = 0; *NT = 0;
 }
 
 }  // namespace dgk
