@@ -109,7 +109,7 @@ void upload_ops(Ctx& c, const RefElem& R, const std::vector<int>& vmap) {
   // MFMA A fragments: k-step s < d*NG covers W_r (r = s % d) at node 4*(s/d) + g for lane group g;
   // later k-steps cover face entry 4*(s - d*NG) + g of -LIFT. Rows beyond Np are zero. Layout
   // [block][k-step][lane]: MBF 16-row blocks (lane l: row 16*mb + l%16, lane group l/16), then NT
-  // 4-row tail pieces for the 4x4x1 MFMA (lane l: row 16*MBF + 4t + l%4, lane group l/16).
+  // 4-row tail pieces for the 4x4 MFMAs (lane l: row 16*MBF + 4t + l%4, lane group l/16).
   int KS = 0, MB = 0, NT = 0;
   mfma_shape(R.dim, R.p, sizeof(T) == 4, c.oplo, &KS, &MB, &NT);
   const int Np = R.Np, NFF = R.Nf * R.Nfp, NG = (Np + 3) / 4, d = R.dim, MBF = NT ? MB - 1 : MB;

@@ -228,15 +228,31 @@ __device__ __forceinline__ Mfma<float>::V mma_bf16(bf16x8 a, bf16x8 b, Mfma<floa
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// 16 blocks of 4x4x1 into an FP32 accumulator; never called for other accumulator types
-template <typename VV, typename TT>
-__device__ __forceinline__ VV mma_4x4x1(TT, TT, VV c) {
-  return c;
-}
+// MFMA for a 4-row piece of the operator (layouts from in-bounds probes on gfx950):
+// FP32 v_mfma_f32_4x4x1_16b_f32, 16 blocks of 4x4 with K = 1, 8 cycles: A lane 4b + i holds row i
+//   of block b, B lane 4b + j column j, register i of lane 4b + j is D_b[i][j]. Block b = 4*kc + cg
+//   takes the k-step's lane group kc and columns 4cg..4cg+3, which is exactly the B operand the lane
+//   holds for the 16x16x4 MFMA; each lane group accumulates its own k residue, summed at the end.
+// FP64 v_mfma_f64_4x4x4_4b_f64, 4 blocks of 4x4 with K = 4, 16 cycles: A lane l holds row l%4,
+//   k = l/16 of block (l%16)/4; B lane l column l%4, k = l/16 of block (l%16)/4; lane L holds
+//   D[row L/16][column L%4] of block (L%16)/4. Blocks = column groups, all 4 k of the k-step.
+// The A fragment is the same for both: lane l holds row l%4 of the piece at lane group l/16.
+template <typename T>
+struct Tail;
 template <>
-__device__ __forceinline__ Mfma<float>::V mma_4x4x1(float a, float b, Mfma<float>::V c) {
-  return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0);
-}
+struct Tail<float> {
+  typedef Mfma<float>::V A;
+  static __device__ __forceinline__ A mma(float a, float b, A c) {
+    return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Tail<double> {
+  typedef double A;
+  static __device__ __forceinline__ A mma(double a, double b, A c) {
+    return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
+  }
+};
 
 template <int D, int P>
 struct MfmaShape {
@@ -248,17 +264,13 @@ struct MfmaShape {
   static constexpr int MB = (NP + 15) / 16;   // 16-row blocks
   static constexpr int RB = NFF > NP ? NFF : NP;
   static constexpr int NLO = (KS + 7) / 8;    // bf16 low-order correction MFMAs per m-block
-  // The last 16-row block holds only RT = Np - 16(MB-1) operator rows. In the FP32 form without
-  // the operator correction, when they fit one 4-row piece (3D orders 1, 3, 4; 2D order 1; 1D up to
-  // order 3), that piece runs on v_mfma_f32_4x4x1_16b_f32 (16 blocks of 4x4, 8 cycles) instead of
-  // one 16x16x4 MFMA (32 cycles) per k-step (two or three pieces measured slower: one more operator
-  // read from LDS per k-step, and at 3D order 5 register spills): block b = 4*kc + cg takes the k-step's lane group kc and columns 4cg..4cg+3, which is
-  // exactly the B operand the lane already holds (lane 16kc + 4cg + j); the four k-residue partial
-  // sums are added across lane groups after the chain. Lane layout (in-bounds probe on gfx950): A
-  // lane 4b + i holds row i of block b, B lane 4b + j column j, accumulator register i of lane
-  // 4b + j is D_b[i][j].
+  // The last 16-row block holds only RT = Np - 16(MB-1) operator rows. Without the FP32 operator
+  // correction, when they fit one 4-row piece (3D orders 1, 3, 4; 2D order 1; 1D up to order 3),
+  // that piece runs on a 4x4 MFMA (Tail: FP32 8 cycles, FP64 16) instead of a 16x16x4 one (FP32 32
+  // cycles, FP64 64) per k-step. Two or three pieces measured slower (one more operator read from
+  // LDS per k-step, and at 3D order 5 register spills).
   static constexpr int RT = NP - 16 * (MB - 1), NT4 = (RT + 3) / 4;
-  static constexpr int tail(bool fp32, bool lo) { return fp32 && !lo && NT4 == 1 ? 1 : 0; }
+  static constexpr int tail(bool /*fp32*/, bool lo) { return !lo && NT4 == 1 ? 1 : 0; }
 };
 
 // Output mapping of the 16x16x4 accumulator: row of register j in lane l (column is l % 16).
@@ -557,11 +569,12 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
         }
       }
     }
-    V acc[MBF ? MBF : 1], acct[NT4 ? NT4 : 1];
+    V acc[MBF ? MBF : 1];
+    typename Tail<T>::A acct[NT4 ? NT4 : 1];
 #pragma unroll
     for (int mb = 0; mb < MBF; ++mb) acc[mb] = V{0, 0, 0, 0};
 #pragma unroll
-    for (int t = 0; t < NT4; ++t) acct[t] = V{0, 0, 0, 0};
+    for (int t = 0; t < NT4; ++t) acct[t] = typename Tail<T>::A{};
     bf16x8 blo;
     // Fragments read from global memory (shapes whose operator does not fit in LDS) are reloaded per
     // tile through an opaque pointer and the node loop is only partly unrolled: otherwise the
@@ -572,7 +585,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
 #pragma unroll
       for (int mb = 0; mb < MBF; ++mb) acc[mb] = Mfma<T>::mma(pAt[(mb * KS + s) * 64], b, acc[mb]);
 #pragma unroll
-      for (int t = 0; t < NT4; ++t) acct[t] = mma_4x4x1(pAt[((MBF + t) * KS + s) * 64], b, acct[t]);
+      for (int t = 0; t < NT4; ++t) acct[t] = Tail<T>::mma(pAt[((MBF + t) * KS + s) * 64], b, acct[t]);
       if constexpr (LO) {
         blo[s % 8] = (__bf16)b;
         if (s % 8 == 7 || s == KS - 1) {
@@ -623,16 +636,22 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
         if (i < NP) sR[AOS ? (el * NP + i) * 4 + q : q * TE * NP + el * NP + i] = acc[mb][j];
       }
 #pragma unroll
-    for (int t = 0; t < NT4; ++t)
+    for (int t = 0; t < NT4; ++t) {
+      if constexpr (sizeof(T) == 4) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        // lane group g holds the partial sum over k-steps' lane group g: add the four
-        T v = acct[t][j];
-        v += __shfl_xor(v, 16);
-        v += __shfl_xor(v, 32);
-        const int i = 16 * MBF + 4 * t + j;
-        if (g == 0 && i < NP) sR[AOS ? (el * NP + i) * 4 + q : q * TE * NP + el * NP + i] = v;
+        for (int j = 0; j < 4; ++j) {
+          // lane group g holds the partial sum over the k-steps' lane group g: add the four
+          T v = acct[t][j];
+          v += __shfl_xor(v, 16);
+          v += __shfl_xor(v, 32);
+          const int i = 16 * MBF + 4 * t + j;
+          if (g == 0 && i < NP) sR[AOS ? (el * NP + i) * 4 + q : q * TE * NP + el * NP + i] = v;
+        }
+      } else {
+        const int i = 16 * MBF + 4 * t + g;  // lane group g holds row g of the piece
+        if (i < NP) sR[AOS ? (el * NP + i) * 4 + q : q * TE * NP + el * NP + i] = (T)acct[t];
       }
+    }
     __syncthreads();
     if (kDefer && !kALds) load_epi();
     // phase 4: time-integration epilogue on 16-byte vectors
