@@ -79,8 +79,8 @@ MshModel build_model(int dim, int p, const std::vector<double>& vxyz, const std:
       if (ns == 1) {
         tag = sup[0].first + 1;
       } else {
-        int64_t vv[4];
-        int ii[4];
+        int64_t vv[4] = {0, 0, 0, 0};
+        int ii[4] = {0, 0, 0, 0};
         for (int k = 0; k < ns; ++k) { vv[k] = sup[k].first; ii[k] = sup[k].second; }
         bool interior = (ns == nvpe);  // nodes strictly inside the element are never shared
         if (interior) {
