@@ -41,6 +41,7 @@ import workloads as wl  # noqa: E402
 METRIC = "DOF-RK-stage updates/s: FP64 plain vs FP32-MFMA batched at stated L2 err"
 UNIT = "DOF-RK-stage updates/s"
 HBM_PEAK = 8.0e12
+MFMA_PEAK = {4: 157.3e12, 8: 78.6e12}  # dense FP32 / FP64 matrix-core peaks of MI355X, flop/s
 FIELDS = ["p", "vx", "vy", "vz"]
 
 
@@ -74,6 +75,15 @@ def algorithmic_bytes(dim, p, K, F, Fb, word, compensated):
     return dof * state + traces + geo
 
 
+def algorithmic_flops(dim, p, K):
+    """Matrix-core flops of one stage launch over K elements: per field and element the product
+    [W_1..W_d | -LIFT] (Np x (d*Np + Nf*Nfp)) times that field's contravariant fluxes and face
+    fluxes, i.e. 2*(d*Np + Nf*Nfp) flops per counted DOF ((dim+1) fields per node). The face flux,
+    the flux contraction with J^-1 and the RK update run on the vector ALU and are not counted."""
+    Np, Nf, Nfp = element_sizes(dim, p)
+    return 2.0 * (dim * Np + Nf * Nfp) * (dim + 1) * K * Np
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` from the rocprofv3 FETCH_SIZE/WRITE_SIZE
     summaries committed under profiles/ (calibrated against a device copy there), or None."""
@@ -91,12 +101,25 @@ def measured_traffic(kernel, workload):
     return None
 
 
-def roofline(bytes_per_launch, stage_s, kernel, workload):
-    ach = bytes_per_launch / stage_s
+def roofline(bytes_per_launch, flops_per_launch, word, stage_s, kernel, workload):
+    """The bound is the roof the kernel's algorithmic intensity (flops per HBM byte) meets first:
+    HBM below the ridge MFMA_PEAK/HBM_PEAK, the matrix cores above it."""
+    peak_f = MFMA_PEAK[word]
+    mfma = flops_per_launch / bytes_per_launch > peak_f / HBM_PEAK
+    ach_b, ach_f = bytes_per_launch / stage_s, flops_per_launch / stage_s
     tr = measured_traffic(kernel, workload)
-    out = {"bound": "hbm", "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-           "frac": round(ach / HBM_PEAK, 4), "traffic": tr["bytes"] if tr else None, "kernel": kernel,
-           "kernel_avg_us": round(stage_s * 1e6, 2), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+    if mfma:
+        out = {"bound": "mfma", "achieved": round(ach_f / 1e12, 2), "peak": peak_f / 1e12, "unit": "TFLOP/s",
+               "frac": round(ach_f / peak_f, 4)}
+    else:
+        out = {"bound": "hbm", "achieved": round(ach_b / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+               "frac": round(ach_b / HBM_PEAK, 4)}
+    out.update({"traffic": tr["bytes"] if tr else None, "kernel": kernel, "kernel_avg_us": round(stage_s * 1e6, 2),
+                "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                "algorithmic_flops_per_launch": int(flops_per_launch),
+                "intensity_flop_per_byte": round(flops_per_launch / bytes_per_launch, 2),
+                "ridge_flop_per_byte": round(peak_f / HBM_PEAK, 2),
+                "algorithmic_GBps": round(ach_b / 1e9, 1), "mfma_TFLOPs": round(ach_f / 1e12, 2)})
     if tr:
         out["traffic_source"] = tr["source"]
         out["traffic_GBps"] = round(tr["bytes"] / stage_s / 1e9, 1)
@@ -122,8 +145,8 @@ def single_leg(m, ph, dt, u0, precision, steps, warmup):
     b = algorithmic_bytes(m.dim, m.order, m.K, info.num_faces, info.num_boundary_faces, word, precision == dgk.FP32)
     stage_s = stage_ms * 1e-3 / (4 * nk)
     return dict(rate=dof * 4 * steps / wall, wall=wall, state=state, total_steps=warmup + steps + nk,
-                roofline=roofline(b, stage_s, "k_stage_mfma<%s,3,3>" % ("double" if word == 8 else "float"),
-                                  "cube3d-p3"),
+                roofline=roofline(b, algorithmic_flops(m.dim, m.order, m.K), word, stage_s,
+                                  "k_stage_mfma<%s,3,3>" % ("double" if word == 8 else "float"), "cube3d-p3"),
                 finite=bool(np.isfinite(state).all()))
 
 
@@ -171,13 +194,15 @@ def batch_leg(steps, warmup, members_checked=8, n=256):
         bl = algorithmic_bytes(g["dim"], g["order"], g["elements"], g["faces"], g["boundary_faces"], 4, True)
         st = ms * 1e-3 / (4 * nk)
         gtab.append(dict(dim=g["dim"], order=g["order"], sims=g["sims"], elements=g["elements"],
-                         dof=(g["dim"] + 1) * g["elements"] * Np, stage_us=round(st * 1e6, 1), bytes=bl, stage_s=st))
+                         dof=(g["dim"] + 1) * g["elements"] * Np, stage_us=round(st * 1e6, 1), bytes=bl,
+                         flops=algorithmic_flops(g["dim"], g["order"], g["elements"]), stage_s=st))
     tot = sum(x["stage_s"] for x in gtab)
     for x in gtab:
         x["share"] = round(x["stage_s"] / tot, 3)
         x["GBps"] = round(x["bytes"] / x["stage_s"] / 1e9, 1)
+        x["TFLOPs"] = round(x["flops"] / x["stage_s"] / 1e12, 2)
     dom = max(gtab, key=lambda x: x["stage_s"])
-    roof = roofline(dom["bytes"], dom["stage_s"],
+    roof = roofline(dom["bytes"], dom["flops"], 4, dom["stage_s"],
                     "k_stage_mfma<float,%d,%d> (largest batch group)" % (dom["dim"], dom["order"]), "ensemble256")
     roof["sum_of_group_stage_us"] = round(tot * 1e6, 1)
     roof["all_groups_GBps"] = round(sum(x["bytes"] for x in gtab) / tot / 1e9, 1)
@@ -199,7 +224,7 @@ def batch_leg(steps, warmup, members_checked=8, n=256):
             errs[FIELDS[q]] = max(errs[FIELDS[q]], float(np.linalg.norm(g32[i][q] - r[q]) / np.linalg.norm(r[q])))
     sub.close()
     for x in gtab:
-        del x["stage_s"], x["bytes"]
+        del x["stage_s"], x["bytes"], x["flops"]
     return dict(rate=dof * 4 * steps / wall, wall=wall, dof=dof, elements=sum(m.K for m in meshes), groups=gtab,
                 roofline=roof, errs=errs, checked=chk, total_steps=total, finite=finite,
                 schedule="parallel" if schedule == dgk.SCHEDULE_PARALLEL else "sequential",

@@ -97,14 +97,15 @@ def test_receivers_fp32_within_bound():
 
 
 def test_batch_receivers_match_oracle():
-    ms = [mixed_mesh(2, 3), mixed_mesh(3, 2), mixed_mesh(2, 3, 7)]
-    dts = [2e-6, 1.5e-6, 1e-6]
+    # members 1 and 3 share a 3D group, so member 3's receivers and state sit at an offset
+    ms = [mixed_mesh(2, 3), mixed_mesh(3, 2), mixed_mesh(2, 3, 7), mixed_mesh(3, 2, 4)]
+    dts = [2e-6, 1.5e-6, 1e-6, 1.2e-6]
     ph = dict(c0=340.0, rho0=1.225, v0=(0.0, 0.0, 0.0))
-    b = dgk.Batch(ms, [ph] * 3, dts, flux_sign=dgk.SIGN_STABLE)
+    b = dgk.Batch(ms, [ph] * 4, dts, flux_sign=dgk.SIGN_STABLE)
     u0 = [smooth_state(m, 5 + i) for i, m in enumerate(ms)]
     for i, u in enumerate(u0):
         b.set_state(i, u)
-    pts = {0: points(2, 4, 1), 2: points(2, 2, 2), 1: points(3, 3, 3)}
+    pts = {0: points(2, 4, 1), 2: points(2, 2, 2), 1: points(3, 3, 3), 3: points(3, 2, 4)}
     for i, p in pts.items():
         b.set_receivers(i, p, 30 + 5 * i)
     b.rk4(30)
