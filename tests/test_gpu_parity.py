@@ -211,6 +211,29 @@ def test_fp32_compensated_state_removes_accumulated_rounding():
     assert comp < 0.5 * plain
 
 
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_fp64_state_round_trip_is_exact(dim):
+    """set_state/get_state move FP64 values unchanged, through field planes (1D/2D) or the
+    node-interleaved 3D device layout, also for batch members at a group offset."""
+    m = mixed_mesh(dim, 3)
+    u = np.random.default_rng(9).standard_normal((4, m.K * m.Np))
+    if dim < 3:
+        u[dim + 1:] = 0.0
+    s = dgk.Solver(m, 340.0, 1.225)
+    s.set_state(u)
+    assert np.array_equal(s.get_state(), u)
+    s.close()
+    m2 = mixed_mesh(dim, 3, 5)
+    u2 = np.random.default_rng(10).standard_normal((4, m2.K * m2.Np))
+    ph = dict(c0=340.0, rho0=1.225, v0=(0.0, 0.0, 0.0))
+    b = dgk.Batch([m, m2], [ph, ph], [1e-6, 2e-6])
+    b.set_state(1, u2)
+    b.set_state(0, u)
+    assert np.array_equal(b.get_state(0), u)
+    assert np.array_equal(b.get_state(1), u2)
+    b.close()
+
+
 def test_fp32_compensated_state_round_trip():
     m = mixed_mesh(3, 3)
     s = dgk.Solver(m, 340.0, 1.225, precision=dgk.FP32)
