@@ -218,7 +218,7 @@ void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* 
   // the thread-per-node kernel implements the plain FP32 form only
   const bool f32mfma = (o ? o->precision : DGK_FP64) == DGK_FP32 && c.variant == kVariantMfma;
   c.comp = f32mfma && !(flags & DGK_FP32_PLAIN_STATE);
-  c.aos = m->dim == 3 && c.variant == kVariantMfma;
+  c.es = c.variant == kVariantMfma ? m->dim + 1 : 1;  // the state starts at zero: dim+1 live fields
   c.oplo = f32mfma && (flags & DGK_FP32_OPERATOR_CORRECTION);
   if (c.precision != DGK_FP64 && c.precision != DGK_FP32) throw InvalidArg("precision must be DGK_FP64 or DGK_FP32");
   if (c.sign_mode != DGK_SIGN_REFERENCE && c.sign_mode != DGK_SIGN_STABLE) throw InvalidArg("bad flux_sign");
@@ -660,7 +660,7 @@ namespace {
 // planes are copied directly; otherwise through the FP64 staging planes and a conversion kernel.
 void upload_planes(Ctx& c, const double* u, void* dst) {
   const size_t row = (size_t)c.K * c.Np * sizeof(double);
-  if (c.precision == DGK_FP64 && !c.aos) {
+  if (c.precision == DGK_FP64 && c.es == 1) {
     DGK_HIP(hipMemcpy2DAsync(dst, c.PS * sizeof(double), u, row, row, 4, hipMemcpyHostToDevice, c.stream));
     return;
   }
@@ -671,7 +671,7 @@ void upload_planes(Ctx& c, const double* u, void* dst) {
 
 void download_planes(Ctx& c, const void* src, double* u) {
   const size_t row = (size_t)c.K * c.Np * sizeof(double);
-  if (c.precision == DGK_FP64 && !c.aos) {
+  if (c.precision == DGK_FP64 && c.es == 1) {
     DGK_HIP(hipMemcpy2DAsync(u, row, src, c.PS * sizeof(double), row, 4, hipMemcpyDeviceToHost, c.stream));
     return;
   }
@@ -727,11 +727,15 @@ dgk_status dgk_set_state(dgk_ctx* c, const double* u) {
       c->graph = nullptr;
     }
     c->nq = nq;
-    // planes beyond nq are never written by the kernels: clear them in every buffer (Euler steps
-    // swap U with a stage buffer that may hold values from an earlier 4-plane state)
-    for (void* p : {c->U, c->A, c->V0, c->V1})
-      DGK_HIP(hipMemsetAsync((char*)p + nq * c->PS * c->tsize, 0, (4 - nq) * c->PS * c->tsize, c->stream));
-    if (c->Ulo) DGK_HIP(hipMemsetAsync(c->Ulo + nq * c->PS, 0, (4 - nq) * c->PS * sizeof(uint16_t), c->stream));
+    if (c->es > 1) {
+      c->es = nq;  // interleaved buffers hold exactly the live fields; the upload below relays U
+    } else {
+      // planes beyond nq are never written by the kernels: clear them in every buffer (Euler steps
+      // swap U with a stage buffer that may hold values from an earlier 4-plane state)
+      for (void* p : {c->U, c->A, c->V0, c->V1})
+        DGK_HIP(hipMemsetAsync((char*)p + nq * c->PS * c->tsize, 0, (4 - nq) * c->PS * c->tsize, c->stream));
+      if (c->Ulo) DGK_HIP(hipMemsetAsync(c->Ulo + nq * c->PS, 0, (4 - nq) * c->PS * sizeof(uint16_t), c->stream));
+    }
     upload_planes(*c, u, c->U);
     DGK_HIP(hipStreamSynchronize(c->stream));
   });
@@ -845,18 +849,27 @@ dgk_status dgk_euler(dgk_ctx* c, int nsteps, double dt, const double* t_start) {
 dgk_status dgk_rhs(dgk_ctx* c, const double* u, double* out, double dt) {
   if (!c || !u || !out) return fail(c, DGK_ERR_INVALID, "NULL argument");
   return guarded(c, [&] {
-    upload_planes(*c, u, c->V1);
-    StageArgs a = base_args(*c);
-    a.dt = dt;
-    a.stage = 4;
-    a.Vin = c->V1;
-    a.Vout = c->V0;
-    a.nq = live_planes(c->dim, u, c->K * c->Np);
-    launch_stage(*c, a, c->stream);
-    download_planes(*c, c->V0, out);
-    DGK_HIP(hipStreamSynchronize(c->stream));
+    const int nq = live_planes(c->dim, u, c->K * c->Np);
+    const int es = c->es;
+    if (es > 1) c->es = nq;  // the operand's own live fields, for this call only
+    try {
+      upload_planes(*c, u, c->V1);
+      StageArgs a = base_args(*c);
+      a.dt = dt;
+      a.stage = 4;
+      a.Vin = c->V1;
+      a.Vout = c->V0;
+      a.nq = nq;
+      launch_stage(*c, a, c->stream);
+      download_planes(*c, c->V0, out);
+      DGK_HIP(hipStreamSynchronize(c->stream));
+    } catch (...) {
+      c->es = es;
+      throw;
+    }
+    c->es = es;
     const int64_t n = c->K * c->Np;  // planes that are not live have a zero right-hand side
-    std::fill(out + a.nq * n, out + 4 * n, 0.0);
+    std::fill(out + nq * n, out + 4 * n, 0.0);
   });
 }
 
@@ -1163,10 +1176,18 @@ dgk_status batch_state_io(dgk_batch* b, int sim, double* u, bool to_device) {
         DGK_HIP(hipGraphExecDestroy(b->graph));
         b->graph = nullptr;
       }
+      if (nq != c.nq && c.es > 1) {
+        // interleaved buffers hold exactly the live fields: relay the whole group's state
+        double* s = dgk::staging(c);
+        uint16_t* lo = c.comp ? c.Ulo : nullptr;
+        dgk::state_to_f64(c, c.U, lo, s, 0, c.K * c.Np);
+        c.es = nq;
+        dgk::state_from_f64(c, s, c.U, lo, 0, c.K * c.Np);
+      }
       c.nq = nq;
     }
     // FP64 field planes are copied directly; otherwise through the FP64 staging planes
-    const bool direct = f64 && !c.aos;
+    const bool direct = f64 && c.es == 1;
     double* s = direct ? nullptr : dgk::staging(c);
     uint16_t* lo = c.comp ? c.Ulo : nullptr;
     if (!to_device && !direct) dgk::state_to_f64(c, c.U, lo, s, off, n);

@@ -93,10 +93,10 @@ struct Ctx {
   uint16_t* Ulo = nullptr;  // FP32 compensated form: bf16 low word of U
   bool comp = false;        // FP32: compensated solution (U, Ulo)
   bool oplo = false;        // FP32: bf16 operator-residual correction on MFMA
-  // State layout of the device buffers U, A, V0, V1 (and Ulo): node-interleaved [Kpad][Np][4] for 3D
-  // contexts on the MFMA kernel, else four field planes [4][Kpad*Np] (plane stride PS). The host
-  // side always exchanges field planes.
-  bool aos = false;
+  // State layout of the device buffers U, A, V0, V1 (and Ulo): on the MFMA kernel node-interleaved
+  // [Kpad][Np][es] with the es = nq live fields; on the thread-per-node kernel (es = 1) four field
+  // planes [4][Kpad*Np] (plane stride PS). The host side always exchanges field planes.
+  int es = 1;
   // Field planes that carry data. In 1D/2D the velocity components beyond dim stay identically
   // zero under the dynamics when they start at zero (no derivative or normal has those
   // components), so their planes are neither read nor written: nq = dim+1. A state with nonzero

@@ -320,7 +320,21 @@ struct MfmaLaunch {
   static constexpr int TE = 4 * WAVES;   // elements per tile
 };
 
-template <typename T, int D, int P, bool LO, bool CS>
+// ES consecutive field values of one node (ES = 4: one vector read), zero beyond ES
+template <int ES, typename T>
+__device__ __forceinline__ void node_fields(const T* p, T f[4]) {
+  if constexpr (ES == 4) {
+    typedef T V4 __attribute__((ext_vector_type(4)));
+    const V4 w = *(const V4*)p;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) f[c] = w[c];
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) f[c] = c < ES ? p[c] : (T)0;
+  }
+}
+
+template <typename T, int D, int P, bool LO, bool CS, int ES>
 __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageArgs a) {
   static_assert(!CS || sizeof(T) == 4, "the compensated state is an FP32 form");
   using S = Sz<D, P>;
@@ -334,13 +348,12 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   constexpr int TE = MfmaLaunch<T, D, P>::TE;
   constexpr int VW = 16 / sizeof(T);                        // values per 16-byte vector
   typedef T VT __attribute__((ext_vector_type(VW)));
-  typedef T V4 __attribute__((ext_vector_type(4)));         // the 4 fields of one node
   constexpr int PV = TE * NP / VW;                          // vectors per field plane of a tile
-  // 3D contexts keep the state node-interleaved ([element][node][field], 4 live fields): a tile is
-  // one contiguous block, a neighbour trace one 4-field load, a node's fields one LDS vector.
-  // 1D/2D keep field planes, so that their dead velocity planes cost no traffic.
-  constexpr bool AOS = D == 3;
-  constexpr int ES = AOS ? 4 : 1;  // state values per element node in a buffer's element stride
+  // ES > 1: the state is node-interleaved, [element][node][field] with the ES live fields (nq),
+  // so a tile is one contiguous block, a neighbour trace one load of ES values and a node's
+  // fields one LDS read. ES = 1: four field planes (the thread-per-node kernel's layout).
+  static_assert(ES == 1 || ES == D + 1 || ES == 4, "ES: 1 (planes) or the live field count");
+  constexpr bool AOS = ES > 1;
   constexpr int NE = (4 * PV + kBlock - 1) / kBlock;        // state vectors per thread
   constexpr int NI = (TE * NFF + kBlock - 1) / kBlock;      // face items per thread
   constexpr int NGEO = (TE * 9 + kBlock - 1) / kBlock;      // geometry values per thread
@@ -376,7 +389,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   const T rc2 = (T)(a.rho0 * a.c0 * a.c0), irho = (T)(1.0 / a.rho0);
   const T v0[3] = {(T)a.v0[0], (T)a.v0[1], (T)a.v0[2]};
   const int stage = a.stage;
-  const int nq = a.nq;
+  const int nq = AOS ? ES : a.nq;  // interleaved buffers hold exactly the live fields
   if (!AOS)
     for (int i = tid + nq * TE * NP; i < 4 * TE * NP; i += kBlock) (&sU[0][0])[i] = (T)0;  // planes that stay zero
   for (int i = tid; i < NF * NPERM * NFP; i += kBlock) sVmap[i] = a.vmap[i];
@@ -438,9 +451,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
       if (inc[t].x >= 0) {
         const int pi = inc[t].x * NP + sVmap[(inc[t].y & 0xffff) * NFP + k];
         if constexpr (AOS) {
-          const V4 w = *byte_at((const V4*)Vin, (uint32_t)pi * (uint32_t)sizeof(V4));
-#pragma unroll
-          for (int c = 0; c < 4; ++c) iqp[t][c] = w[c];
+          node_fields<ES>(byte_at(Vin, (uint32_t)pi * (uint32_t)(ES * sizeof(T))), iqp[t]);
         } else {
 #pragma unroll
           for (int c = 0; c < 4; ++c)
@@ -529,8 +540,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
       const int on = e * NP + sFmask[jk];
       T qm[4];
       if constexpr (AOS) {
-        const V4 w = *(const V4*)(&sU[0][0] + on * 4);
-        for (int c = 0; c < 4; ++c) qm[c] = w[c];
+        node_fields<ES>(&sU[0][0] + on * ES, qm);
       } else {
         for (int c = 0; c < 4; ++c) qm[c] = sU[c][on];
       }
@@ -604,11 +614,12 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
       T p = 0, vx = 0, vy = 0, vz = 0;
       if (n < NP) {
         if constexpr (AOS) {
-          const V4 w = *(const V4*)(&sU[0][0] + (el * NP + n) * 4);
-          p = w[0];
-          vx = w[1];
-          vy = w[2];
-          vz = w[3];
+          T f[4];
+          node_fields<ES>(&sU[0][0] + (el * NP + n) * ES, f);
+          p = f[0];
+          vx = f[1];
+          vy = f[2];
+          vz = f[3];
         } else {
           p = sU[0][el * NP + n];
           vx = sU[1][el * NP + n];
@@ -633,7 +644,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int i = 16 * mb + d_row<T>(lane, j);
-        if (i < NP) sR[AOS ? (el * NP + i) * 4 + q : q * TE * NP + el * NP + i] = acc[mb][j];
+        if (i < NP && q < nq) sR[AOS ? (el * NP + i) * ES + q : q * TE * NP + el * NP + i] = acc[mb][j];
       }
 #pragma unroll
     for (int t = 0; t < NT4; ++t) {
@@ -645,11 +656,11 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
           v += __shfl_xor(v, 16);
           v += __shfl_xor(v, 32);
           const int i = 16 * MBF + 4 * t + j;
-          if (g == 0 && i < NP) sR[AOS ? (el * NP + i) * 4 + q : q * TE * NP + el * NP + i] = v;
+          if (g == 0 && i < NP && q < nq) sR[AOS ? (el * NP + i) * ES + q : q * TE * NP + el * NP + i] = v;
         }
       } else {
         const int i = 16 * MBF + 4 * t + g;  // lane group g holds row g of the piece
-        if (i < NP) sR[AOS ? (el * NP + i) * 4 + q : q * TE * NP + el * NP + i] = (T)acct[t];
+        if (i < NP && q < nq) sR[AOS ? (el * NP + i) * ES + q : q * TE * NP + el * NP + i] = (T)acct[t];
       }
     }
     __syncthreads();
@@ -755,14 +766,28 @@ __global__ void k_finite(const T* __restrict__ x, int64_t n, int* __restrict__ f
 }
 
 // FP64 field planes (plane stride PS; element nodes [off, off+n) of each plane) -> a state buffer in
-// the context's precision and layout (es = 4: node-interleaved, else planes), with the bf16 low word
-// of the FP32 remainder when lo is given; and back.
+// the context's precision and layout (es > 1: node-interleaved with es fields, the planes beyond es
+// being zero; es = 1: planes), with the bf16 low word of the FP32 remainder when lo is given; and
+// back (planes beyond es come back as zero).
+__device__ __forceinline__ void field_node(int64_t i, int64_t n, int64_t off, int es, int64_t PS, int64_t* q,
+                                           int64_t* node, int64_t* di) {
+  if (es > 1) {
+    *q = i & 3;
+    *node = off + (i >> 2);
+    *di = *node * es + *q;
+  } else {
+    *q = i / n;
+    *node = off + (i - *q * n);
+    *di = *q * PS + *node;
+  }
+}
 template <typename T>
 __global__ void k_from_f64(const double* __restrict__ s, int64_t PS, int64_t off, int64_t n, T* __restrict__ d,
                            uint16_t* __restrict__ lo, int es) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 4 * n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t q = es == 4 ? (i & 3) : i / n, node = off + (es == 4 ? i >> 2 : i - q * n);
-    const int64_t di = es == 4 ? node * 4 + q : q * PS + node;
+    int64_t q, node, di;
+    field_node(i, n, off, es, PS, &q, &node, &di);
+    if (es > 1 && q >= es) continue;
     const double x = s[q * PS + node];
     if constexpr (sizeof(T) == 8) {
       d[di] = x;
@@ -777,9 +802,9 @@ template <typename T>
 __global__ void k_to_f64(const T* __restrict__ d, const uint16_t* __restrict__ lo, int64_t PS, int64_t off, int64_t n,
                          double* __restrict__ s, int es) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 4 * n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t q = es == 4 ? (i & 3) : i / n, node = off + (es == 4 ? i >> 2 : i - q * n);
-    const int64_t di = es == 4 ? node * 4 + q : q * PS + node;
-    s[q * PS + node] = (double)d[di] + (lo ? (double)bf16_bits_to_f32(lo[di]) : 0.0);
+    int64_t q, node, di;
+    field_node(i, n, off, es, PS, &q, &node, &di);
+    s[q * PS + node] = (es > 1 && q >= es) ? 0.0 : (double)d[di] + (lo ? (double)bf16_bits_to_f32(lo[di]) : 0.0);
   }
 }
 
@@ -811,12 +836,12 @@ void launch_lds(StageArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((k_stage_lds<T, D, P>), dim3(8 * bpx), dim3(kBlock), 0, s, a);
 }
 
-template <typename T, int D, int P, bool LO, bool CS>
-void launch_mfma(StageArgs& a, hipStream_t s, int grid_pct) {
+template <typename T, int D, int P, bool LO, bool CS, int ES>
+void launch_mfma_es(StageArgs& a, hipStream_t s, int grid_pct) {
   constexpr int TE = MfmaLaunch<T, D, P>::TE, KB = MfmaLaunch<T, D, P>::KB;
   static int occ = 0;
   if (!occ) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stage_mfma<T, D, P, LO, CS>, KB, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stage_mfma<T, D, P, LO, CS, ES>, KB, 0) != hipSuccess ||
         occ < 1)
       occ = 1;
   }
@@ -825,18 +850,28 @@ void launch_mfma(StageArgs& a, hipStream_t s, int grid_pct) {
   const int64_t per_xcd_cap = (int64_t)(num_cus() / 8) * occ * std::max(grid_pct, 1) / 100;
   int bpx = (int)std::min<int64_t>(a.tiles_per_xcd, std::max<int64_t>(per_xcd_cap, 1));
   if (bpx < 1) bpx = 1;
-  hipLaunchKernelGGL((k_stage_mfma<T, D, P, LO, CS>), dim3(8 * bpx), dim3(KB), 0, s, a);
+  hipLaunchKernelGGL((k_stage_mfma<T, D, P, LO, CS, ES>), dim3(8 * bpx), dim3(KB), 0, s, a);
+}
+
+// MFMA contexts keep the state node-interleaved with their live fields: ES = 4 in 3D, and in
+// 1D/2D dim+1 (the extra velocity components identically zero) or 4.
+template <typename T, int D, int P, bool LO, bool CS>
+void launch_mfma(StageArgs& a, hipStream_t s, int grid_pct, int es) {
+  if constexpr (D < 3)
+    if (es == D + 1) return launch_mfma_es<T, D, P, LO, CS, D + 1>(a, s, grid_pct);
+  if (es != 4) throw InvalidArg("stage kernel: interleaved state with an unsupported field count");
+  launch_mfma_es<T, D, P, LO, CS, 4>(a, s, grid_pct);
 }
 
 template <typename T, int D, int P>
 void launch_variant(Ctx& c, StageArgs& a, hipStream_t s) {
   if (c.variant == kVariantLds) return launch_lds<T, D, P>(a, s);
   if constexpr (std::is_same<T, float>::value) {
-    if (c.comp && c.oplo) return launch_mfma<T, D, P, true, true>(a, s, c.grid_pct);
-    if (c.comp) return launch_mfma<T, D, P, false, true>(a, s, c.grid_pct);
-    if (c.oplo) return launch_mfma<T, D, P, true, false>(a, s, c.grid_pct);
+    if (c.comp && c.oplo) return launch_mfma<T, D, P, true, true>(a, s, c.grid_pct, c.es);
+    if (c.comp) return launch_mfma<T, D, P, false, true>(a, s, c.grid_pct, c.es);
+    if (c.oplo) return launch_mfma<T, D, P, true, false>(a, s, c.grid_pct, c.es);
   }
-  launch_mfma<T, D, P, false, false>(a, s, c.grid_pct);
+  launch_mfma<T, D, P, false, false>(a, s, c.grid_pct, c.es);
 }
 
 template <typename T>
@@ -863,10 +898,10 @@ void launch_source(Ctx& c, const Ctx::Src& src, double value, hipStream_t st) {
   unsigned nb = (unsigned)((src.n + 255) / 256);
   if (c.precision == DGK_FP64)
     hipLaunchKernelGGL(k_source<double>, dim3(nb), dim3(256), 0, st, (double*)c.U, (uint16_t*)nullptr, src.idx,
-                       src.n, value, c.aos ? 4 : 1);
+                       src.n, value, c.es);
   else
     hipLaunchKernelGGL(k_source<float>, dim3(nb), dim3(256), 0, st, (float*)c.U, (uint16_t*)(c.comp ? c.Ulo : nullptr),
-                       src.idx, src.n, (float)value, c.aos ? 4 : 1);
+                       src.idx, src.n, (float)value, c.es);
   DGK_HIP(hipGetLastError());
 }
 
@@ -875,11 +910,11 @@ void launch_record(Ctx& c, hipStream_t st) {
   if (c.precision == DGK_FP64)
     hipLaunchKernelGGL(k_record<double>, dim3(1), dim3(256), 0, st, (const double*)c.U, (const uint16_t*)nullptr,
                        c.rec.base, (const double*)c.rec.w, c.rec.n, c.Np, (double*)c.rec.out, c.rec.counter,
-                       c.rec.cap, c.aos ? 4 : 1);
+                       c.rec.cap, c.es);
   else
     hipLaunchKernelGGL(k_record<float>, dim3(1), dim3(256), 0, st, (const float*)c.U,
                        (const uint16_t*)(c.comp ? c.Ulo : nullptr), c.rec.base, (const float*)c.rec.w, c.rec.n,
-                       c.Np, (float*)c.rec.out, c.rec.counter, c.rec.cap, c.aos ? 4 : 1);
+                       c.Np, (float*)c.rec.out, c.rec.counter, c.rec.cap, c.es);
   DGK_HIP(hipGetLastError());
 }
 
@@ -898,7 +933,7 @@ bool check_finite(Ctx& c) {
 }
 
 void state_from_f64(Ctx& c, const double* s, void* dst, uint16_t* lo, int64_t off, int64_t n) {
-  const int es = c.aos ? 4 : 1;
+  const int es = c.es;
   if (c.precision == DGK_FP64)
     hipLaunchKernelGGL(k_from_f64<double>, dim3(1024), dim3(256), 0, c.stream, s, c.PS, off, n, (double*)dst,
                        (uint16_t*)nullptr, es);
@@ -908,7 +943,7 @@ void state_from_f64(Ctx& c, const double* s, void* dst, uint16_t* lo, int64_t of
 }
 
 void state_to_f64(Ctx& c, const void* src, const uint16_t* lo, double* s, int64_t off, int64_t n) {
-  const int es = c.aos ? 4 : 1;
+  const int es = c.es;
   if (c.precision == DGK_FP64)
     hipLaunchKernelGGL(k_to_f64<double>, dim3(1024), dim3(256), 0, c.stream, (const double*)src,
                        (const uint16_t*)nullptr, c.PS, off, n, s, es);
