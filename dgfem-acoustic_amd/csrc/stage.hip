@@ -15,8 +15,9 @@
 // The face terms are exact at face nodes because every integrand is a polynomial of degree
 // <= 2p on straight faces.
 //
-// Data layout in HBM is the reference's state layout u[eq][element][node] in 1D/2D and the
-// node-interleaved u[element][node][eq] for 3D contexts on the MFMA kernel; a workgroup
+// Data layout in HBM is the reference's state layout u[eq][element][node] for the
+// thread-per-node kernel and the node-interleaved u[element][node][eq] (live fields only) for the
+// MFMA kernel; a workgroup
 // stages a tile of TE consecutive elements in LDS (coalesced loads of TE*Np contiguous
 // values per equation, or of one contiguous block), gathers neighbour traces through the
 // device-built face table, and walks tiles in an XCD-aware order (each XCD sweeps one
@@ -403,7 +404,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   // Per-thread offsets that do not depend on the tile (every tile is full: the element count is
   // padded to a multiple of TE and padding elements carry zero state and geometry).
   // Field planes: plane*PS + r (add e0*NP) and plane*TE*NP + r. Node-interleaved: the tile is one
-  // block, v*VW in both (add e0*NP*4). In values.
+  // block, v*VW in both (add e0*NP*ES). In values.
   uint32_t voff[NE];  // offset of the thread's state vectors in a state buffer
   int loff[NE];       // and in the LDS tile
 #pragma unroll
@@ -717,7 +718,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   }
 }
 
-// idx holds pressure-plane offsets e*Np + node; es = 4 for node-interleaved state, else 1
+// idx holds pressure-plane offsets e*Np + node; es = the interleaved field count, 1 for planes
 template <typename T>
 __global__ void k_source(T* __restrict__ U, uint16_t* __restrict__ Ulo, const int* __restrict__ idx, int64_t n,
                          T value, int es) {
@@ -729,8 +730,8 @@ __global__ void k_source(T* __restrict__ U, uint16_t* __restrict__ Ulo, const in
   }
 }
 
-// One workgroup: every thread interpolates receivers r = tid, tid+256, ... from the pressure plane
-// and writes row *counter of the record; the row index is read before and advanced after the
+// One workgroup: every thread interpolates receivers r = tid, tid+256, ... from the pressure field
+// (value (e*Np + node)*es of the state buffer) and writes row *counter of the record; the row index is read before and advanced after the
 // writes, so a captured graph records consecutive rows on every replay. Rows beyond cap are dropped.
 template <typename T>
 __global__ void k_record(const T* __restrict__ U, const uint16_t* __restrict__ Ulo, const int* __restrict__ base,
