@@ -350,18 +350,17 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   constexpr int VW = 16 / sizeof(T);                        // values per 16-byte vector
   typedef T VT __attribute__((ext_vector_type(VW)));
   constexpr int PV = TE * NP / VW;                          // vectors per field plane of a tile
-  // ES > 1: the state is node-interleaved, [element][node][field] with the ES live fields (nq),
-  // so a tile is one contiguous block, a neighbour trace one load of ES values and a node's
-  // fields one LDS read. ES = 1: four field planes (the thread-per-node kernel's layout).
-  static_assert(ES == 1 || ES == D + 1 || ES == 4, "ES: 1 (planes) or the live field count");
-  constexpr bool AOS = ES > 1;
+  // The state is node-interleaved, [element][node][field] with the ES live fields (nq), so a tile
+  // is one contiguous block of ES*PV vectors, a neighbour trace one load of ES values and a node's
+  // fields one LDS read; the LDS tiles use the same layout.
+  static_assert(ES == D + 1 || ES == 4, "ES: the live field count");
   constexpr int NE = (4 * PV + kBlock - 1) / kBlock;        // state vectors per thread
   constexpr int NI = (TE * NFF + kBlock - 1) / kBlock;      // face items per thread
   constexpr int NGEO = (TE * 9 + kBlock - 1) / kBlock;      // geometry values per thread
   constexpr int NFC = (TE * NF + kBlock - 1) / kBlock;      // element faces per thread
   typedef typename Mfma<T>::V V;
   __shared__ __attribute__((aligned(32))) T sU[4][TE * NP];
-  __shared__ __attribute__((aligned(16))) T sB[4][TE * RB];  // face fluxes, then results as [4][TE*NP]
+  __shared__ __attribute__((aligned(16))) T sB[4][TE * RB];  // face fluxes, then results in sU's layout
   __shared__ T sN[TE * NF][4];
   __shared__ T sG[TE][9];
   __shared__ int sVmap[NF * NPERM * NFP];
@@ -377,7 +376,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, col = lane & 15, el = wave * 4 + (col >> 2), q = col & 3;
-  const int PS = (int)a.PS;  // plane stride Kpad*Np; 4*PS*sizeof(T) < 2^32 is checked at context creation
+  // 32-bit byte offsets: a buffer holds 4*PS*sizeof(T) < 2^32 bytes (checked at context creation)
   const T* __restrict__ Vin = (const T*)a.Vin;
   T* __restrict__ Vout = (T*)a.Vout;
   T* __restrict__ U = (T*)a.U;
@@ -390,9 +389,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   const T rc2 = (T)(a.rho0 * a.c0 * a.c0), irho = (T)(1.0 / a.rho0);
   const T v0[3] = {(T)a.v0[0], (T)a.v0[1], (T)a.v0[2]};
   const int stage = a.stage;
-  const int nq = AOS ? ES : a.nq;  // interleaved buffers hold exactly the live fields
-  if (!AOS)
-    for (int i = tid + nq * TE * NP; i < 4 * TE * NP; i += kBlock) (&sU[0][0])[i] = (T)0;  // planes that stay zero
+  constexpr int nq = ES;  // the buffers hold exactly the live fields
   for (int i = tid; i < NF * NPERM * NFP; i += kBlock) sVmap[i] = a.vmap[i];
   for (int i = tid; i < NF * NFP; i += kBlock) sFmask[i] = a.fmask[i];
   if (kALds)
@@ -402,16 +399,15 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
     for (int i = tid; i < MB * NLO * 64; i += kBlock) sAlo[i] = ((const bf16x8*)a.Alo)[i];
 
   // Per-thread offsets that do not depend on the tile (every tile is full: the element count is
-  // padded to a multiple of TE and padding elements carry zero state and geometry).
-  // Field planes: plane*PS + r (add e0*NP) and plane*TE*NP + r. Node-interleaved: the tile is one
-  // block, v*VW in both (add e0*NP*ES). In values.
-  uint32_t voff[NE];  // offset of the thread's state vectors in a state buffer
-  int loff[NE];       // and in the LDS tile
+  // padded to a multiple of TE and padding elements carry zero state and geometry): the thread's
+  // state vectors sit at v*VW values into the tile, in a state buffer (add e0*NP*ES) and in LDS.
+  uint32_t voff[NE];
+  int loff[NE];
 #pragma unroll
   for (int t = 0; t < NE; ++t) {
-    const int v = tid + t * kBlock, qq = v / PV, r = (v - qq * PV) * VW;
-    voff[t] = AOS ? (uint32_t)(v * VW) : (uint32_t)(qq * PS + r);
-    loff[t] = AOS ? v * VW : qq * TE * NP + r;
+    const int v = tid + t * kBlock;
+    voff[t] = (uint32_t)(v * VW);
+    loff[t] = v * VW;
   }
 
   // Software pipeline: while tile t is processed, tile t+1's state, geometry and face geometry,
@@ -451,13 +447,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
       const int k = ((tid + t * kBlock) % NFF) % NFP;
       if (inc[t].x >= 0) {
         const int pi = inc[t].x * NP + sVmap[(inc[t].y & 0xffff) * NFP + k];
-        if constexpr (AOS) {
-          node_fields<ES>(byte_at(Vin, (uint32_t)pi * (uint32_t)(ES * sizeof(T))), iqp[t]);
-        } else {
-#pragma unroll
-          for (int c = 0; c < 4; ++c)
-            iqp[t][c] = c < nq ? *byte_at(Vin, (uint32_t)(c * PS + pi) * (uint32_t)sizeof(T)) : (T)0;
-        }
+        node_fields<ES>(byte_at(Vin, (uint32_t)pi * (uint32_t)(ES * sizeof(T))), iqp[t]);
       } else {
 #pragma unroll
         for (int c = 0; c < 4; ++c) iqp[t][c] = (T)0;
@@ -540,11 +530,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
       const T fs = sN[ej][3];
       const int on = e * NP + sFmask[jk];
       T qm[4];
-      if constexpr (AOS) {
-        node_fields<ES>(&sU[0][0] + on * ES, qm);
-      } else {
-        for (int c = 0; c < 4; ++c) qm[c] = sU[c][on];
-      }
+      node_fields<ES>(&sU[0][0] + on * ES, qm);
       const T vn = v0[0] * n[0] + v0[1] * n[1] + v0[2] * n[2];
       const T unm = qm[1] * n[0] + qm[2] * n[1] + qm[3] * n[2];
       const bool inter = nc.x >= 0, refl = nc.x == -2;
@@ -614,19 +600,12 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
       const int n = 4 * m + g;
       T p = 0, vx = 0, vy = 0, vz = 0;
       if (n < NP) {
-        if constexpr (AOS) {
-          T f[4];
-          node_fields<ES>(&sU[0][0] + (el * NP + n) * ES, f);
-          p = f[0];
-          vx = f[1];
-          vy = f[2];
-          vz = f[3];
-        } else {
-          p = sU[0][el * NP + n];
-          vx = sU[1][el * NP + n];
-          vy = sU[2][el * NP + n];
-          vz = sU[3][el * NP + n];
-        }
+        T f[4];
+        node_fields<ES>(&sU[0][0] + (el * NP + n) * ES, f);
+        p = f[0];
+        vx = f[1];
+        vy = f[2];
+        vz = f[3];
       }
 #pragma unroll
       for (int r = 0; r < D; ++r) step(m * D + r, alpha[r] * p + beta[r][0] * vx + beta[r][1] * vy + beta[r][2] * vz);
@@ -645,7 +624,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int i = 16 * mb + d_row<T>(lane, j);
-        if (i < NP && q < nq) sR[AOS ? (el * NP + i) * ES + q : q * TE * NP + el * NP + i] = acc[mb][j];
+        if (i < NP && q < nq) sR[(el * NP + i) * ES + q] = acc[mb][j];
       }
 #pragma unroll
     for (int t = 0; t < NT4; ++t) {
@@ -657,11 +636,11 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
           v += __shfl_xor(v, 16);
           v += __shfl_xor(v, 32);
           const int i = 16 * MBF + 4 * t + j;
-          if (g == 0 && i < NP && q < nq) sR[AOS ? (el * NP + i) * ES + q : q * TE * NP + el * NP + i] = v;
+          if (g == 0 && i < NP && q < nq) sR[(el * NP + i) * ES + q] = v;
         }
       } else {
         const int i = 16 * MBF + 4 * t + g;  // lane group g holds row g of the piece
-        if (i < NP && q < nq) sR[AOS ? (el * NP + i) * ES + q : q * TE * NP + el * NP + i] = (T)acct[t];
+        if (i < NP && q < nq) sR[(el * NP + i) * ES + q] = (T)acct[t];
       }
     }
     __syncthreads();
