@@ -151,3 +151,40 @@ def test_batch_high_order_members():
         e32 = rel(out[dgk.FP32][i][0], out[dgk.FP64][i][0])
         print(f"  member {i} dim={m.dim} p={m.order}: fp64 {e64:.2e}, fp32 p {e32:.2e}", flush=True)
         assert e64 <= 1e-12 and e32 <= 1e-5
+
+
+@pytest.mark.parametrize("precision", [dgk.FP64, dgk.FP32])
+def test_batch_group_live_fields_change(precision):
+    """Two 2D members in one group: the group's live fields follow its members' states (3 while
+    every vz is zero, 4 once one is not), relaying the interleaved group state; each member
+    still matches its own oracle run after every change."""
+    m0 = dgk.gen_rect(6, 5, 1.0, 0.8, 3, [1, 2, 1, 2], "1:Wall;2:Abs", conf(BCS))
+    m1 = dgk.gen_rect(5, 4, 1.0, 0.8, 3, [2, 2, 1, 1], "1:Wall;2:Abs", conf(BCS))
+    ph = dict(c0=340.0, rho0=1.225, v0=(20.0, -10.0, 0.0))
+    dts = [2e-6, 1.5e-6]
+    b = dgk.Batch([m0, m1], [ph, ph], dts, precision=precision, flux_sign=dgk.SIGN_STABLE)
+    assert len(b.groups()) == 1
+    rng = np.random.default_rng(17)
+    u0 = np.zeros((4, m0.K * m0.Np))
+    u0[:3] = rng.standard_normal((3, m0.K * m0.Np))        # vz = 0: three live fields
+    u1 = rng.standard_normal((4, m1.K * m1.Np))            # vz != 0: four
+    u1b = u1.copy()
+    u1b[3] = 0.0
+    tol = 1e-12 if precision == dgk.FP64 else 1e-5
+    oracles = [ob.Oracle(m, ph["c0"], ph["rho0"], v0=ph["v0"], sign_mode=ob.SIGN_STABLE) for m in (m0, m1)]
+    b.set_state(0, u0)
+    b.set_state(1, u1)  # 3 -> 4 fields: member 0's data is relaid
+    b.rk4(10)
+    r0 = oracles[0].step(u0, 10, dts[0])
+    r1 = oracles[1].step(u1, 10, dts[1])
+    assert rel(b.get_state(0), r0) <= tol
+    assert rel(b.get_state(1), r1) <= tol
+    g0 = b.get_state(0)
+    b.set_state(1, u1b)  # 4 -> 3 fields again: member 0's data is relaid back
+    assert rel(b.get_state(0), g0) <= (0.0 if precision == dgk.FP64 else 1e-7)
+    b.rk4(10)
+    assert rel(b.get_state(0), oracles[0].step(r0, 10, dts[0])) <= tol
+    got1 = b.get_state(1)
+    assert rel(got1, oracles[1].step(u1b, 10, dts[1])) <= tol
+    assert not got1[3].any()
+    b.close()
