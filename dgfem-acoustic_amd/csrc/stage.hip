@@ -349,7 +349,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   constexpr int TE = MfmaLaunch<T, D, P>::TE;
   constexpr int VW = 16 / sizeof(T);                        // values per 16-byte vector
   typedef T VT __attribute__((ext_vector_type(VW)));
-  constexpr int PV = TE * NP / VW;                          // vectors per field plane of a tile
+  constexpr int PV = TE * NP / VW;                          // vectors per field of a tile (ES*PV in all)
   // The state is node-interleaved, [element][node][field] with the ES live fields (nq), so a tile
   // is one contiguous block of ES*PV vectors, a neighbour trace one load of ES values and a node's
   // fields one LDS read; the LDS tiles use the same layout.
@@ -697,7 +697,8 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   }
 }
 
-// idx holds pressure-plane offsets e*Np + node; es = the interleaved field count, 1 for planes
+// idx holds node offsets e*Np + node; the pressure sits at (e*Np + node)*es, es being the
+// interleaved field count (1 for field planes)
 template <typename T>
 __global__ void k_source(T* __restrict__ U, uint16_t* __restrict__ Ulo, const int* __restrict__ idx, int64_t n,
                          T value, int es) {
