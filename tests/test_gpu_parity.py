@@ -312,10 +312,12 @@ def test_room_full_size_fp32_vs_fp64_with_receivers():
     assert rel(out[dgk.FP32][1], out[dgk.FP64][1]) <= 1e-4
 
 
-@pytest.mark.parametrize("dim,p", [(2, 5), (3, 4), (3, 5)])
-def test_fp32_rk4_high_order_against_oracle(dim, p):
-    """FP32 form at the orders with their own workgroup shapes (3D P5: 8-wave workgroups)."""
-    m = mixed_mesh(dim, p, 4)
+@pytest.mark.parametrize("dim,p", CASES)
+def test_fp32_rk4_every_shape_against_oracle(dim, p):
+    """FP32 form on every (dim, order): the interleaved state with 2, 3 and 4 fields, operators
+    whose rows all sit in the 4x4 tail MFMA (1D orders 1-3, 2D/3D order 1), tails beside 16x16x4
+    blocks (3D orders 3 and 4) and the 8-wave workgroups of 3D order 5."""
+    m = mixed_mesh(dim, p, 4 if dim == 3 else None)
     x = m.node_xyz
     u0 = np.zeros((4, m.K * m.Np))
     u0[0] = np.exp(-((x - x.mean(axis=0)) ** 2).sum(axis=1) / 0.05)
