@@ -108,8 +108,9 @@ void upload_ops(Ctx& c, const RefElem& R, const std::vector<int>& vmap) {
   DGK_HIP(hipMemcpy(c.ops.vmap, vmap.data(), vmap.size() * sizeof(int), hipMemcpyHostToDevice));
   // MFMA A fragments: k-step s < d*NG covers W_r (r = s % d) at node 4*(s/d) + g for lane group g;
   // later k-steps cover face entry 4*(s - d*NG) + g of -LIFT. Rows beyond Np are zero. Layout
-  // [block][k-step][lane]: MBF 16-row blocks (lane l: row 16*mb + l%16, lane group l/16), then NT
-  // 4-row tail pieces for the 4x4 MFMAs (lane l: row 16*MBF + 4t + l%4, lane group l/16).
+  // [block][k-step][lane]: MBF 16-row blocks (lane l: row 16*mb + l%16, lane group l/16), then the
+  // NT 4-row tail pieces for the 4x4 MFMAs as [piece][k-step][lane group g][row r] (16 values; lane
+  // l reads row l%4 at lane group l/16).
   int KS = 0, MB = 0, NT = 0;
   mfma_shape(R.dim, R.p, sizeof(T) == 4, c.oplo, &KS, &MB, &NT);
   const int Np = R.Np, NFF = R.Nf * R.Nfp, NG = (Np + 3) / 4, d = R.dim, MBF = NT ? MB - 1 : MB;
@@ -126,13 +127,14 @@ void upload_ops(Ctx& c, const RefElem& R, const std::vector<int>& vmap) {
     }
     return v;
   };
-  std::vector<T> A((size_t)(MBF + NT) * KS * 64, (T)0);
-  for (int mb = 0; mb < MBF + NT; ++mb)
+  std::vector<T> A((size_t)(MBF * 64 + NT * 16) * KS, (T)0);
+  for (int mb = 0; mb < MBF; ++mb)
     for (int s = 0; s < KS; ++s)
-      for (int l = 0; l < 64; ++l) {
-        const int i = mb < MBF ? 16 * mb + (l % 16) : 16 * MBF + 4 * (mb - MBF) + (l % 4);
-        A[((size_t)mb * KS + s) * 64 + l] = (T)coef(i, s, l / 16);
-      }
+      for (int l = 0; l < 64; ++l) A[((size_t)mb * KS + s) * 64 + l] = (T)coef(16 * mb + (l % 16), s, l / 16);
+  for (int t = 0; t < NT; ++t)
+    for (int s = 0; s < KS; ++s)
+      for (int v = 0; v < 16; ++v)
+        A[(size_t)MBF * KS * 64 + ((size_t)t * KS + s) * 16 + v] = (T)coef(16 * MBF + 4 * t + v % 4, s, v / 4);
   c.ops.Amf = c.alloc(A.size() * sizeof(T));
   DGK_HIP(hipMemcpy(c.ops.Amf, A.data(), A.size() * sizeof(T), hipMemcpyHostToDevice));
   if (sizeof(T) == 4 && c.oplo) {

@@ -311,12 +311,13 @@ __device__ __forceinline__ uint32_t f32_to_bf16_bits(float f) {
   return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;  // round to nearest even (finite inputs)
 }
 
-// Workgroup shape: 4 waves (16-element tiles) by default. FP32 3D order 5 keeps its 64.5 KB
-// operator in LDS, which leaves room for one workgroup per CU; it runs 8 waves (32-element tiles)
-// so that two waves per SIMD share that one copy.
+// Workgroup shape: 4 waves (16-element tiles) by default. FP32 3D order 5 (64.5 KB operator) and
+// FP64 3D order 4 (48.4 KB operator, 25 KB tile buffers per 16 elements) leave room for one
+// workgroup per CU; they run 8 waves (32-element tiles) so that two waves per SIMD share the one
+// operator copy.
 template <typename T, int D, int P>
 struct MfmaLaunch {
-  static constexpr int WAVES = (sizeof(T) == 4 && D == 3 && P == 5) ? 8 : 4;
+  static constexpr int WAVES = ((sizeof(T) == 4 && D == 3 && P == 5) || (sizeof(T) == 8 && D == 3 && P == 4)) ? 8 : 4;
   static constexpr int KB = 64 * WAVES;  // threads per workgroup
   static constexpr int TE = 4 * WAVES;   // elements per tile
 };
@@ -345,7 +346,9 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   constexpr int NG = M::NG, NFK = M::NFK, KS = M::KS, MB = M::MB, RB = M::RB, NLO = M::NLO;
   constexpr int NT4 = M::tail(sizeof(T) == 4, LO);  // 4-row tail pieces (0: all blocks 16x16x4)
   constexpr int MBF = NT4 ? MB - 1 : MB;             // 16x16x4 row blocks
-  constexpr int NFRAG = MBF + NT4;                   // A fragments per k-step
+  // A fragments per k-step: MBF 16x16x4 blocks of 64 lane values, and the tail piece of 16
+  // distinct values (row l%4 at lane group l/16; lanes 4 apart in l/4 read the same value)
+  constexpr int NAV = (MBF * 64 + NT4 * 16) * KS;    // operator values
   constexpr int TE = MfmaLaunch<T, D, P>::TE;
   constexpr int VW = 16 / sizeof(T);                        // values per 16-byte vector
   typedef T VT __attribute__((ext_vector_type(VW)));
@@ -369,8 +372,8 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   // of a gfx950 CU (fewer resident workgroups, but no per-k-step global loads).
   constexpr size_t kLdsRest = sizeof(T) * (4 * TE * NP + 4 * TE * RB + TE * NF * 4 + TE * 9) +
                               sizeof(int) * (NF * NPERM * NFP + NF * NFP) + (LO ? 16 * MB * NLO * 64 : 0);
-  constexpr bool kALds = NFRAG * KS * 64 * sizeof(T) + kLdsRest <= 160 * 1024 - 512;
-  __shared__ T sA[kALds ? NFRAG * KS * 64 : 1];
+  constexpr bool kALds = NAV * sizeof(T) + kLdsRest <= 160 * 1024 - 512;
+  __shared__ T sA[kALds ? NAV : 1];
   __shared__ bf16x8 sAlo[LO ? MB * NLO * 64 : 1];
   T* const sR = &sB[0][0];
 
@@ -393,8 +396,9 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   for (int i = tid; i < NF * NPERM * NFP; i += kBlock) sVmap[i] = a.vmap[i];
   for (int i = tid; i < NF * NFP; i += kBlock) sFmask[i] = a.fmask[i];
   if (kALds)
-    for (int i = tid; i < NFRAG * KS * 64; i += kBlock) sA[i] = Af[i];
+    for (int i = tid; i < NAV; i += kBlock) sA[i] = Af[i];
   const T* __restrict__ pA = (kALds ? (const T*)sA : Af) + lane;
+  const T* __restrict__ pT = (kALds ? (const T*)sA : Af) + MBF * KS * 64 + 4 * (lane >> 4) + (lane & 3);
   if (LO)
     for (int i = tid; i < MB * NLO * 64; i += kBlock) sAlo[i] = ((const bf16x8*)a.Alo)[i];
 
@@ -577,12 +581,13 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
     // tile through an opaque pointer and the node loop is only partly unrolled: otherwise the
     // compiler hoists every fragment of the operator into registers and spills.
     const T* pAt = pA;
-    if (!kALds) asm volatile("" : "+v"(pAt));
+    const T* pTt = pT;
+    if (!kALds) asm volatile("" : "+v"(pAt), "+v"(pTt));
     auto step = [&](const int s, const T b) {
 #pragma unroll
       for (int mb = 0; mb < MBF; ++mb) acc[mb] = Mfma<T>::mma(pAt[(mb * KS + s) * 64], b, acc[mb]);
 #pragma unroll
-      for (int t = 0; t < NT4; ++t) acct[t] = Tail<T>::mma(pAt[((MBF + t) * KS + s) * 64], b, acct[t]);
+      for (int t = 0; t < NT4; ++t) acct[t] = Tail<T>::mma(pTt[(t * KS + s) * 16], b, acct[t]);
       if constexpr (LO) {
         blo[s % 8] = (__bf16)b;
         if (s % 8 == 7 || s == KS - 1) {
