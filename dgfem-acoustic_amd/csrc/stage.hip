@@ -311,13 +311,14 @@ __device__ __forceinline__ uint32_t f32_to_bf16_bits(float f) {
   return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;  // round to nearest even (finite inputs)
 }
 
-// Workgroup shape: 4 waves (16-element tiles) by default. FP32 3D order 5 (64.5 KB operator) and
-// FP64 3D order 4 (48.4 KB operator, 25 KB tile buffers per 16 elements) leave room for one
-// workgroup per CU; they run 8 waves (32-element tiles) so that two waves per SIMD share the one
-// operator copy.
+// Workgroup shape: 4 waves (16-element tiles) by default. 3D orders 4 and 5 run 8 waves (32-element
+// tiles) so that two waves per SIMD share one operator copy in LDS: FP32 order 5 (64.5 KB
+// operator) and FP64 order 4 (48.4 KB operator, 50 KB of tile buffers per 16 elements) have room
+// for one workgroup per CU; FP32 order 4 (24 KB operator) is 2-4% faster than with two 4-wave
+// workgroups. FP64 order 5 reads its operator from global memory and keeps 4 waves.
 template <typename T, int D, int P>
 struct MfmaLaunch {
-  static constexpr int WAVES = ((sizeof(T) == 4 && D == 3 && P == 5) || (sizeof(T) == 8 && D == 3 && P == 4)) ? 8 : 4;
+  static constexpr int WAVES = (D == 3 && (P == 5 || P == 4) && !(sizeof(T) == 8 && P == 5)) ? 8 : 4;
   static constexpr int KB = 64 * WAVES;  // threads per workgroup
   static constexpr int TE = 4 * WAVES;   // elements per tile
 };
