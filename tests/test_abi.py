@@ -91,3 +91,23 @@ def test_curved_element_is_rejected():
     xyz[3, 1] += 0.05  # bend the first edge node of element 0
     st, msg, h = _create(2, 2, m.K, xyz, m.node_tags)
     assert st == 2 and "curved" in msg and not h
+
+
+def test_shipped_tuning_table_covers_every_shape_and_loads(tmp_path):
+    """tuning.txt has one valid row per (dim, order, precision) and the library accepts it; a row
+    naming an unknown variant or an out-of-range launch shape is rejected (no device use)."""
+    ensure_device_lib()
+    rows = [l.split() for l in open(dgk.TUNING_FILE) if l.strip() and not l.startswith("#")]
+    keys = sorted((int(r[0]), int(r[1]), int(r[2])) for r in rows)
+    assert keys == sorted((d, p, q) for d in (1, 2, 3) for p in range(1, 6) for q in (dgk.FP64, dgk.FP32))
+    for r in rows:
+        assert int(r[3]) in (dgk.VARIANT_LDS, dgk.VARIANT_MFMA) and 1 <= int(r[4]) <= 10000 and int(r[5]) in (0, 1)
+        if int(r[2]) == dgk.FP32:
+            assert int(r[3]) == dgk.VARIANT_MFMA  # the compensated FP32 form runs on the MFMA kernel
+    L = dgk.device()
+    assert L.dgk_load_tuning(dgk.TUNING_FILE.encode()) == 0
+    for bad in ("3 3 1 9 100 0\n", "3 3 1 2 0 0\n", "3 3 1 2 100 2\n", "4 3 1 2 100 0\n", "3 3 1\n"):
+        f = tmp_path / "bad.txt"
+        f.write_text(bad)
+        assert L.dgk_load_tuning(str(f).encode()) == dgk.DGK_ERR_INVALID, bad
+    assert L.dgk_load_tuning(dgk.TUNING_FILE.encode()) == 0
