@@ -364,7 +364,10 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   constexpr int NFC = (TE * NF + kBlock - 1) / kBlock;      // element faces per thread
   typedef typename Mfma<T>::V V;
   __shared__ __attribute__((aligned(32))) T sU[4][TE * NP];
-  __shared__ __attribute__((aligned(16))) T sB[4][TE * RB];  // face fluxes, then results in sU's layout
+  // face fluxes [element][field][RB], then the results in sU's layout. A face k-step's 64 lanes read
+  // words c*RB + g + const (c < 16, g < 4): for 4-byte values that is 64 distinct LDS banks whenever
+  // RB is 4 times an odd number (3D orders 4 and 5: 60, 84)
+  __shared__ __attribute__((aligned(16))) T sB[4 * TE * RB];
   __shared__ T sN[TE * NF][4];
   __shared__ T sG[TE][9];
   __shared__ int sVmap[NF * NPERM * NFP];
@@ -376,7 +379,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   constexpr bool kALds = NAV * sizeof(T) + kLdsRest <= 160 * 1024 - 512;
   __shared__ T sA[kALds ? NAV : 1];
   __shared__ bf16x8 sAlo[LO ? MB * NLO * 64 : 1];
-  T* const sR = &sB[0][0];
+  T* const sR = sB;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, col = lane & 15, el = wave * 4 + (col >> 2), q = col & 3;
@@ -516,7 +519,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
     const bool more = valid(it + bpx);
     if (more) prefetch((xcd * tpx + it + bpx) * TE);
     __syncthreads();
-    // phase 2: nodal normal numerical fluxes scaled by sJ/|detJ| -> sB[q][e*RB + jk]. All three face
+    // phase 2: nodal normal numerical fluxes scaled by sJ/|detJ| -> sB[(e*4 + q)*RB + jk]. All three face
     // kinds share one branch-free form (no divergence between interior and boundary face nodes):
     //   fl = A (F(qm) + F(qp)).n-with-speed-vne + H (qm - qp),   F(q).n = (vne p + rc2 v.n, vne v + p/rho n)
     // interior    qp = neighbour trace, A = 1/2, vne = v0.n, H = s c0/2      (Rusanov)
@@ -552,7 +555,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
       for (int c = 0; c < 3; ++c)
         fl[1 + c] = A * ((vne * qm[1 + c] + qm[0] * irho * n[c]) + (vne * qpv[c] + qp0 * irho * n[c])) +
                     H * (qm[1 + c] - qpv[c]);
-      for (int qq = 0; qq < 4; ++qq) sB[qq][e * RB + jk] = fs * fl[qq];
+      for (int qq = 0; qq < 4; ++qq) sB[(e * 4 + qq) * RB + jk] = fs * fl[qq];
     }
     if (more) gather();  // next tile's neighbour traces, in flight during the MFMA phase
     __syncthreads();
@@ -622,7 +625,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
 #pragma unroll
     for (int f = 0; f < NFK; ++f) {
       const int jk = 4 * f + g;
-      step(D * NG + f, jk < NFF ? sB[q][el * RB + jk] : (T)0);
+      step(D * NG + f, jk < NFF ? sB[(el * 4 + q) * RB + jk] : (T)0);
     }
     __syncthreads();  // face fluxes consumed; sB now receives the results in the LDS tile's layout
 #pragma unroll
