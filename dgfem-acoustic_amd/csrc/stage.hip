@@ -498,9 +498,10 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
     // the operator in global memory load them just before the epilogue instead
     VT pu[NE], pa[NE];
     uint2 plo[CS ? NE : 1];
-    // u_n and the accumulator are read and written once per stage per element, far apart in time
-    // from their next use: their loads and stores carry the non-temporal hint, which leaves the
-    // caches to the neighbour traces and stage inputs (1-3% per stage on the HBM-bound shapes)
+    // u_n and the accumulator are touched once per stage per element, and on large meshes more
+    // than the caches hold passes before their next use: their loads and stores carry the
+    // non-temporal hint, which leaves the caches to the neighbour traces and stage inputs (1-3% per
+    // stage on the HBM-bound shapes)
     auto load_epi = [&]() {
     if (stage >= 1 && stage <= 3) {
 #pragma unroll
