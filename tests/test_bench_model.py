@@ -51,3 +51,22 @@ def test_roofline_takes_the_roof_met_first(word, flops, bound):
         assert r["unit"] == "GB/s" and r["peak"] == 8000.0
         assert abs(r["achieved"] - 1000.0) < 1e-6  # 1 GB in 1 ms
     assert abs(r["frac"] - r["achieved"] / r["peak"]) < 1e-3
+
+
+@pytest.mark.parametrize("kernel,workload,dim,p,K,word,comp", [
+    ("k_stage_mfma<double,3,3>", "cube3d-p3", 3, 3, 196608, 8, False),
+    ("k_stage_mfma<float,3,3>", "cube3d-p3", 3, 3, 196608, 4, True),
+    ("k_stage_mfma<float,3,5> (largest batch group)", "ensemble256", 3, 5, None, 4, True),
+])
+def test_measured_traffic_is_found_for_every_roofline_kernel(kernel, workload, dim, p, K, word, comp):
+    """bench.py's roofline.traffic comes from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summaries: each kernel the bench line can name has a row there, and the measured HBM bytes per
+    launch lie within 40% of the algorithmic count for the cube (whose face count is known here)."""
+    tr = bench.measured_traffic(kernel, workload)
+    assert tr is not None and tr["source"].startswith("profiles/") and tr["bytes"] > 0
+    if K is not None:
+        # the cube: 32^3 Kuhn cells x 6 tets, every face interior except the 6 * 2 * 32^2 on the walls
+        nbf = 6 * 2 * 32 * 32
+        nf = (4 * K + nbf) // 2
+        alg = bench.algorithmic_bytes(dim, p, K, nf, nbf, word, comp)
+        assert 0.6 * alg < tr["bytes"] < 1.4 * alg
