@@ -24,7 +24,6 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
-sys.path.insert(0, os.path.join(os.path.dirname(HERE), "tests"))
 os.environ["DGK_NO_TUNING"] = "1"  # time the untuned defaults and the candidates explicitly
 
 import dgk  # noqa: E402

@@ -32,7 +32,6 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
-sys.path.insert(0, os.path.join(os.path.dirname(HERE), "tests"))
 
 import dgk  # noqa: E402
 import surrogate  # noqa: E402

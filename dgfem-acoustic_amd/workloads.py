@@ -1,4 +1,5 @@
-"""Workload definitions shared by tests and bench.py (meshes, physics, initial states).
+"""Workload definitions shared by bench.py, the autotuner, the surrogate room driver and the tests
+(meshes, physics, initial states).
 
 Time steps use a CFL number of 0.1 relative to the DG node spacing, dt = 0.1 h_min /
 (c (2p+1)): with dt = 0.1 h/c the order-3 schemes are unstable under RK4 (2D limit about
