@@ -16,6 +16,17 @@
 #include <unordered_map>
 #include <vector>
 
+// Mutation hooks for the oracle's own pin tests (tests/test_oracle_pins.py builds copies of this
+// file with a planted mistake and checks that the pins reject them). The shipped build uses the
+// reference's values: the absorbing boundary matrix RKR = 1/4 [[c, rho c^2 n^T], [n/rho, c n n^T]]
+// (src/Mesh.cpp:397-415) and the mean flow v0 as given (src/Mesh.cpp:577-591).
+#ifndef ORC_ABS_COEF
+#define ORC_ABS_COEF 0.25
+#endif
+#ifndef ORC_V0_SIGN
+#define ORC_V0_SIGN 1.0
+#endif
+
 namespace {
 
 typedef long double ld;
@@ -647,10 +658,11 @@ void setup(orc_ctx* c, const int* grefl, const int64_t* const* gnodes, const int
     for (int g = 0; g < Gf; ++g) {
       const double* n = &c->fNormals[(f * Gf + g) * 3];
       const double cc = c->c0, r = c->rho0;
-      c->RKR[f * Gf + g] = {0.25 * cc, 0.25 * cc * cc * r * n[0], 0.25 * cc * cc * r * n[1], 0.25 * cc * cc * r * n[2],
-                            0.25 * n[0] / r, 0.25 * cc * n[0] * n[0], 0.25 * cc * n[0] * n[1], 0.25 * cc * n[0] * n[2],
-                            0.25 * n[1] / r, 0.25 * cc * n[1] * n[0], 0.25 * cc * n[1] * n[1], 0.25 * cc * n[1] * n[2],
-                            0.25 * n[2] / r, 0.25 * cc * n[2] * n[0], 0.25 * cc * n[2] * n[1], 0.25 * cc * n[2] * n[2]};
+      const double a = ORC_ABS_COEF;
+      c->RKR[f * Gf + g] = {a * cc, a * cc * cc * r * n[0], a * cc * cc * r * n[1], a * cc * cc * r * n[2],
+                            a * n[0] / r, a * cc * n[0] * n[0], a * cc * n[0] * n[1], a * cc * n[0] * n[2],
+                            a * n[1] / r, a * cc * n[1] * n[0], a * cc * n[1] * n[1], a * cc * n[1] * n[2],
+                            a * n[2] / r, a * cc * n[2] * n[0], a * cc * n[2] * n[1], a * cc * n[2] * n[2]};
     }
   }
   // mass matrices and inverses (src/Mesh.cpp:440-466)
@@ -915,8 +927,9 @@ void num_step_factorized(orc_ctx* c, const double* u, double* out, double dt, do
         for (int q = 0; q < 4; ++q) nodal[q][i] = o[q];
       } else {
         double un = n[0] * qm[1] + n[1] * qm[2] + n[2] * qm[3];
-        nodal[0][i] = 0.25 * cc * qm[0] + 0.25 * cc * cc * r * un;
-        for (int k = 0; k < 3; ++k) nodal[1 + k][i] = 0.25 * n[k] / r * qm[0] + 0.25 * cc * n[k] * un;
+        const double a = ORC_ABS_COEF;
+        nodal[0][i] = a * cc * qm[0] + a * cc * cc * r * un;
+        for (int k = 0; k < 3; ++k) nodal[1 + k][i] = a * n[k] / r * qm[0] + a * cc * n[k] * un;
       }
     }
     double sJ = c->fDet[f * c->Gs];
@@ -985,7 +998,7 @@ orc_ctx* orc_create(int dim, int order, int64_t K, const double* node_xyz, const
     c->Np = dim == 1 ? order + 1 : dim == 2 ? (order + 1) * (order + 2) / 2 : (order + 1) * (order + 2) * (order + 3) / 6;
     if (c->Np > 64) throw std::runtime_error("order too high for the oracle");
     c->c0 = c0; c->rho0 = rho0;
-    for (int k = 0; k < 3; ++k) c->v0[k] = v0 ? v0[k] : 0.0;
+    for (int k = 0; k < 3; ++k) c->v0[k] = v0 ? ORC_V0_SIGN * v0[k] : 0.0;
     c->sign_mode = sign_mode;
     c->nthreads = nthreads > 0 ? nthreads : omp_get_max_threads();
     c->elNodeTags.assign(node_tags, node_tags + K * c->Np);

@@ -22,11 +22,16 @@
  *                   matrices applied transposed, nested face region);
  *   ORC_FACTORIZED  the same discrete operator factorised for affine
  *                   elements (reference-element matrices x per-element scalars).
- * Pins (tests/test_oracle.py, tests/test_reference_element.py): reference matrices against
- * exact rationals from Gmsh's node orderings (tests/golden/reference_matrices.json) and closed
- * forms; literal = factorized to 1e-13; hashed = quadratic face search; one RK4 step = the
- * degree-4 Taylor polynomial of the assembled operator, Euler = I + A; mass conservation,
- * energy decay and steady constants with reflecting walls; the first-node BC rule.
+ * Pins (tests/test_oracle.py, tests/test_oracle_pins.py, tests/test_reference_element.py):
+ * reference matrices against exact rationals from Gmsh's node orderings
+ * (tests/golden/reference_matrices.json) and closed forms; literal = factorized to 1e-13; hashed =
+ * quadratic face search; one RK4 step = the degree-4 Taylor polynomial of the assembled operator,
+ * Euler = I + A; the whole P1 operator (interior, reflecting and absorbing faces, v0 = 0 and
+ * v0 != 0) against an independent closed-form assembly whose absorbing flux is 1/2 A_n^+ from an
+ * eigendecomposition; exact mean-flow solutions (1D d'Alembert, 2D/3D Gaussian blob) converging
+ * at order >= p + 1/2; mass conservation, energy decay with reflecting, absorbing and mixed
+ * walls, steady constants with and without v0; the first-node BC rule. Mutant builds
+ * (ORC_ABS_COEF, ORC_V0_SIGN in oracle.cpp) are rejected by those pins.
  * Parity unpinned: the reference's interior-penalty sign for 3D order >= 2, which follows
  * Gmsh's (unavailable) quadrature point order -- ORC_SIGN_REFERENCE takes the first point of
  * the collapsed Gauss-Legendre rule used here; Gmsh's own face-node order for order >= 4

@@ -17,31 +17,40 @@ SIGN_REFERENCE, SIGN_STABLE = 0, 1
 _lib = None
 
 
+def load(path):
+    """Load an oracle library build (the shipped one, or a mutant built by the pin tests)."""
+    if not os.path.exists(path):
+        raise RuntimeError(f"oracle library missing: {path} (run `make -C oracle`)")
+    L = ctypes.CDLL(path)
+    _declare(L)
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
-        L = ctypes.CDLL(LIB_PATH)
-        P = ctypes.POINTER
-        L.orc_create.restype = ctypes.c_void_p
-        L.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, P(ctypes.c_double), P(ctypes.c_int64),
-                                 ctypes.c_int, P(ctypes.c_int), P(P(ctypes.c_int64)), P(ctypes.c_int64),
-                                 ctypes.c_double, ctypes.c_double, P(ctypes.c_double), ctypes.c_int, ctypes.c_int,
-                                 ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
-        L.orc_destroy.argtypes = [ctypes.c_void_p]
-        L.orc_sizes.argtypes = [ctypes.c_void_p, P(ctypes.c_int64)]
-        L.orc_connectivity.argtypes = [ctypes.c_void_p] + [P(ctypes.c_int32)] * 5
-        L.orc_face_normals.argtypes = [ctypes.c_void_p, P(ctypes.c_double)]
-        L.orc_mass_inverse.argtypes = [ctypes.c_void_p, P(ctypes.c_double)]
-        L.orc_rhs.argtypes = [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_double), ctypes.c_double, ctypes.c_int]
-        L.orc_step.argtypes = [ctypes.c_void_p, P(ctypes.c_double), ctypes.c_int, ctypes.c_double, P(ctypes.c_double),
-                               ctypes.c_int, P(ctypes.c_double), ctypes.c_int, ctypes.c_int]
-        L.orc_ref_matrices.argtypes = [ctypes.c_int, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double),
-                                       P(ctypes.c_double)]
-        L.orc_ref_nodes.argtypes = [ctypes.c_int, ctypes.c_int, P(ctypes.c_double)]
-        _lib = L
+        _lib = load(LIB_PATH)
     return _lib
+
+
+def _declare(L):
+    P = ctypes.POINTER
+    L.orc_create.restype = ctypes.c_void_p
+    L.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, P(ctypes.c_double), P(ctypes.c_int64),
+                             ctypes.c_int, P(ctypes.c_int), P(P(ctypes.c_int64)), P(ctypes.c_int64),
+                             ctypes.c_double, ctypes.c_double, P(ctypes.c_double), ctypes.c_int, ctypes.c_int,
+                             ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    L.orc_destroy.argtypes = [ctypes.c_void_p]
+    L.orc_sizes.argtypes = [ctypes.c_void_p, P(ctypes.c_int64)]
+    L.orc_connectivity.argtypes = [ctypes.c_void_p] + [P(ctypes.c_int32)] * 5
+    L.orc_face_normals.argtypes = [ctypes.c_void_p, P(ctypes.c_double)]
+    L.orc_mass_inverse.argtypes = [ctypes.c_void_p, P(ctypes.c_double)]
+    L.orc_rhs.argtypes = [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_double), ctypes.c_double, ctypes.c_int]
+    L.orc_step.argtypes = [ctypes.c_void_p, P(ctypes.c_double), ctypes.c_int, ctypes.c_double, P(ctypes.c_double),
+                           ctypes.c_int, P(ctypes.c_double), ctypes.c_int, ctypes.c_int]
+    L.orc_ref_matrices.argtypes = [ctypes.c_int, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double),
+                                   P(ctypes.c_double)]
+    L.orc_ref_nodes.argtypes = [ctypes.c_int, ctypes.c_int, P(ctypes.c_double)]
 
 
 def _p(a, t=ctypes.c_double):
@@ -52,8 +61,8 @@ class Oracle:
     """CPU restatement of the reference solver on one mesh (see oracle/oracle.h)."""
 
     def __init__(self, mesh, c0, rho0, v0=(0, 0, 0), sign_mode=SIGN_REFERENCE, nthreads=0, literal_setup=False,
-                 factorized_only=False):
-        L = lib()
+                 factorized_only=False, library=None):
+        L = self.L = library or lib()
         self.mesh = mesh
         self.dim, self.order, self.K, self.Np = mesh.dim, mesh.order, mesh.K, mesh.Np
         xyz = np.ascontiguousarray(mesh.node_xyz, dtype=np.float64)
@@ -79,7 +88,7 @@ class Oracle:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_destroy(self.h)
+            self.L.orc_destroy(self.h)
             self.h = None
 
     def connectivity(self):
@@ -89,24 +98,24 @@ class Oracle:
         nbr = np.zeros(2 * F, np.int32)
         f2e = np.zeros(F * Nfp * 2, np.int32)
         bc = np.zeros(F, np.int32)
-        lib().orc_connectivity(self.h, *[_p(a, ctypes.c_int32) for a in (elF, elO, nbr, f2e, bc)])
+        self.L.orc_connectivity(self.h, *[_p(a, ctypes.c_int32) for a in (elF, elO, nbr, f2e, bc)])
         return dict(elFIds=elF.reshape(K, Nf), elFOrientation=elO.reshape(K, Nf), fNbr=nbr.reshape(F, 2),
                     fNToElNIds=f2e.reshape(F, Nfp, 2), fBC=bc)
 
     def face_normals(self):
         out = np.zeros((self.F, 3))
-        lib().orc_face_normals(self.h, _p(out))
+        self.L.orc_face_normals(self.h, _p(out))
         return out
 
     def mass_inverse(self):
         out = np.zeros((self.K, self.Np, self.Np))
-        lib().orc_mass_inverse(self.h, _p(out))
+        self.L.orc_mass_inverse(self.h, _p(out))
         return out
 
     def rhs(self, u, dt, mode=FACTORIZED):
         u = np.ascontiguousarray(u, dtype=np.float64)
         out = np.zeros_like(u)
-        if lib().orc_rhs(self.h, _p(u), _p(out), float(dt), int(mode)):
+        if self.L.orc_rhs(self.h, _p(u), _p(out), float(dt), int(mode)):
             raise RuntimeError("oracle: literal evaluation needs the literal setup")
         return out
 
@@ -114,7 +123,7 @@ class Oracle:
         u = np.array(u, dtype=np.float64, copy=True, order="C")
         ts = None if t_steps is None else np.ascontiguousarray(t_steps, dtype=np.float64)
         src = np.zeros((0, 9)) if sources is None else np.ascontiguousarray(sources, dtype=np.float64).reshape(-1, 9)
-        rc = lib().orc_step(self.h, _p(u), int(nsteps), float(dt), None if ts is None else _p(ts), len(src),
+        rc = self.L.orc_step(self.h, _p(u), int(nsteps), float(dt), None if ts is None else _p(ts), len(src),
                             _p(src), int(mode), int(euler))
         if rc:
             raise RuntimeError("oracle: literal evaluation needs the literal setup")
