@@ -274,31 +274,100 @@ def small_leg(no_cpu, steps=1000):
     return out
 
 
-def cpu_baseline(meshes, phys, dts, pulses, members, steps=2):
-    """Literal (reference loop structure) oracle on a bounded sample: `steps` RK4 steps of each member."""
+def host_cpu():
+    """Model name and logical CPU count of the host (from /proc/cpuinfo)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
+
+
+def smallest_member_per_shape(n=256):
+    """Index of the smallest member (by drawn element count) of every (dim, order) of the ensemble."""
+    best = {}
+    for i in range(n):
+        dim, K, p = wl.ensemble_draw(i, n=n)[:3]
+        if (dim, p) not in best or K < best[(dim, p)][0]:
+            best[(dim, p)] = (K, i)
+    return sorted(i for _, i in best.values())
+
+
+def cpu_baseline(meshes, steps=1):
+    """The oracle on a bounded sample of the ensemble, all host threads: `steps` RK4 steps of one
+    member per (dim, order) -- the smallest of its shape, drawn again with about 3000 (2D), 1500
+    (3D orders 2-3) or 600 (3D orders 4-5) elements (same seed, shape, sides, walls and pulse), about
+    10 s of host time in all -- with its literal path (the reference's loop
+    structure, quadrature-point arrays and OpenMP layout) and its factorized path. The value is the
+    rate at which the whole ensemble would run: the per-(dim, order) rates weighted by the
+    ensemble's unknowns of that shape (total unknown-stages over the summed time they would take)."""
     import oracle_binding as ob
 
     threads = os.cpu_count() or 1
-    t_lit = t_fac = 0.0
-    dof_total = 0
+    members = smallest_member_per_shape()
+    sm, sp, sd, su = [], [], [], []
     for i in members:
-        m, ph = meshes[i], phys[i]
-        u = wl.ensemble_state(m, pulses[i])
+        dim, _, p = wl.ensemble_draw(i)[:3]
+        k = 3000 if dim == 2 else 1500 if p <= 3 else 600
+        for a, b in zip((sm, sp, sd, su), wl.ensemble(n=256, members=[i], k_target=k, threads=1)):
+            a.extend(b)
+    rates = {}
+    t_lit = 0.0
+    for m, ph, dt, pu in zip(sm, sp, sd, su):
+        u = wl.ensemble_state(m, pu)
         o = ob.Oracle(m, ph["c0"], ph["rho0"], v0=ph["v0"], sign_mode=ob.SIGN_STABLE, nthreads=threads)
         t0 = time.perf_counter()
-        o.step(u, steps, dts[i], mode=ob.LITERAL)
-        t_lit += time.perf_counter() - t0
+        o.step(u, steps, dt, mode=ob.LITERAL)
+        tl = time.perf_counter() - t0
         t0 = time.perf_counter()
-        o.step(u, steps, dts[i], mode=ob.FACTORIZED)
-        t_fac += time.perf_counter() - t0
+        o.step(u, steps, dt, mode=ob.FACTORIZED)
+        tf = time.perf_counter() - t0
         del o
-        dof_total += (m.dim + 1) * m.K * m.Np * steps
-    desc = ", ".join(f"member {i} ({meshes[i].dim}D p{meshes[i].order} K={meshes[i].K})" for i in members)
-    return {"value": dof_total * 4 / t_lit, "unit": UNIT, "cores": threads, "kind": "port",
-            "sample": f"{steps} RK4 steps each of {desc} with the oracle's literal path (the reference's "
-                      f"loop structure and OpenMP layout), {t_lit:.1f} s; factorized_value: the oracle's "
-                      f"factorized path on the same sample",
-            "factorized_value": dof_total * 4 / t_fac}
+        w = (m.dim + 1) * m.K * m.Np * steps * 4
+        rates[(m.dim, m.order)] = (w / tl, w / tf)
+        t_lit += tl
+    share = {}
+    for m in meshes:
+        share[(m.dim, m.order)] = share.get((m.dim, m.order), 0) + (m.dim + 1) * m.K * m.Np
+    tot = sum(share.values())
+    lit = tot / sum(share[k] / rates[k][0] for k in share)
+    fac = tot / sum(share[k] / rates[k][1] for k in share)
+    desc = ", ".join(f"member {i} ({m.dim}D p{m.order} K={m.K})" for i, m in zip(members, sm))
+    return {"value": lit, "unit": UNIT, "cores": threads, "kind": "port", "host_cpu": host_cpu(),
+            "sample": f"{steps} RK4 steps each of {desc} (one member per dim/order) with the oracle's literal "
+                      f"path (the reference's loop structure and OpenMP layout), {t_lit:.1f} s; value: the "
+                      f"per-shape rates weighted by the ensemble's unknowns per shape; factorized_value: the "
+                      f"oracle's factorized path, same weighting",
+            "factorized_value": fac,
+            "per_shape": {f"{d}D p{p}": {"literal": round(r[0], 1), "factorized": round(r[1], 1)}
+                          for (d, p), r in sorted(rates.items())}}
+
+
+def cube_cpu(steps=2, n=12):
+    """The oracle beside the FP64 cube leg: `steps` RK4 steps of the same generator at n^3 cells
+    (the full 32^3 cube takes about a minute per step on the literal path), literal and factorized
+    paths, all host threads."""
+    import oracle_binding as ob
+
+    threads = os.cpu_count() or 1
+    m, ph, dt, u0 = wl.cube_3d(n=n)
+    o = ob.Oracle(m, ph["c0"], ph["rho0"], v0=ph["v0"], sign_mode=ob.SIGN_STABLE, nthreads=threads)
+    w = (m.dim + 1) * m.K * m.Np * steps * 4
+    t0 = time.perf_counter()
+    o.step(u0, steps, dt, mode=ob.LITERAL)
+    tl = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.step(u0, steps, dt, mode=ob.FACTORIZED)
+    tf = time.perf_counter() - t0
+    del o
+    return {"value": w / tl, "unit": UNIT, "cores": threads, "kind": "port", "host_cpu": host_cpu(),
+            "sample": f"first {steps} RK4 steps of the cube generator at {n}^3 cells ({m.K} tets, order 3), "
+                      f"literal path {tl:.1f} s",
+            "factorized_value": w / tf}
 
 
 def main():
@@ -336,6 +405,10 @@ def main():
             res[name] = single_leg(m, ph, dt, u0, prec, args.steps, args.warmup)
             log(f"{name} cube: {res[name]['rate']:.4g} DOF-stage/s, stage kernel "
                 f"{res[name]['roofline']['kernel_avg_us']} us ({res[name]['roofline']['achieved']} GB/s)")
+        if "fp64" in legs and not args.no_cpu:
+            res["fp64"]["cpu"] = cube_cpu()
+            log(f"fp64 cube: oracle literal {res['fp64']['cpu']['value']:.4g}, factorized "
+                f"{res['fp64']['cpu']['factorized_value']:.4g} DOF-stage/s on {res['fp64']['cpu']['cores']} threads")
     if "c1" in legs:
         res["c1"] = small_leg(args.no_cpu)
     if "batch" in legs:
@@ -371,7 +444,8 @@ def main():
     if "fp64" in res:
         r = res["fp64"]
         out["fp64_plain"] = {"value": r["rate"], "ms_per_step": r["wall"] * 1e3 / args.steps, "dtype": "fp64",
-                             "workload": cube_cfg, "roofline": r["roofline"], "finite": r["finite"]}
+                             "workload": cube_cfg, "roofline": r["roofline"], "finite": r["finite"],
+                             "cpu_baseline": r.get("cpu")}
         if "batch" in res:
             out["fp32_batched_over_fp64_plain"] = res["batch"]["rate"] / r["rate"]
     if "fp32" in res:
@@ -385,7 +459,7 @@ def main():
     out["cpu_baseline"] = None
     if not args.no_cpu and "batch" in res:
         log("cpu baseline: literal oracle on a sample of the ensemble, all host threads")
-        out["cpu_baseline"] = cpu_baseline(*res["batch"]["sample"], members=[0, 1, 6, 134, 135])
+        out["cpu_baseline"] = cpu_baseline(res["batch"]["sample"][0])
     print(json.dumps(out))
 
 

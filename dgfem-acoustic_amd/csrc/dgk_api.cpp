@@ -20,9 +20,15 @@ void Ctx::free_receivers() {
   rec = Rec{};
 }
 
+void Ctx::drop_graphs() {
+  for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+  graphs.clear();
+}
+
 Ctx::~Ctx() {
   free_receivers();
-  if (graph) (void)hipGraphExecDestroy(graph);
+  drop_graphs();
+  if (tdev) (void)hipFree(tdev);
   for (auto& s : sources)
     if (s.idx) (void)hipFree(s.idx);
   for (void* p : allocs) (void)hipFree(p);
@@ -195,9 +201,9 @@ void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* 
   if (m->dim < 1 || m->dim > 3) throw Unsupported("element dimension must be 1, 2 or 3");
   if (m->order < 1 || m->order > 5) throw Unsupported("element order must be 1..5");
   if (m->num_elements <= 0 || !m->node_xyz || !m->node_tags) throw InvalidArg("empty mesh");
-  const int64_t tsz = (o && o->precision == DGK_FP32) ? 4 : 8;
-  if (4 * (m->num_elements + kTilePad) * num_nodes(m->dim, m->order) * tsz >= ((int64_t)1 << 32))
-    throw Unsupported("a state buffer (4*K*Np values) must stay below 4 GiB (32-bit device offsets)");
+  // element and element-face indices are 32-bit on the device
+  if ((m->num_elements + kTilePad) * (m->dim + 1) >= ((int64_t)1 << 31))
+    throw Unsupported("more than 2^31 element faces");
   if (!(ph->c0 > 0) || !(ph->rho0 > 0)) throw InvalidArg("c0 and rho0 must be positive");
   c.dim = m->dim;
   c.p = m->order;
@@ -206,6 +212,7 @@ void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* 
   c.sign_mode = o ? o->flux_sign : DGK_SIGN_REFERENCE;
   c.device = o ? o->device : 0;
   c.variant = o ? o->variant : 0;
+  c.addr64 = o && o->addr64;
   Tuned tuned;
   const bool have_tuning = find_tuning(m->dim, m->order, o ? o->precision : DGK_FP64, &tuned);
   if (have_tuning) {
@@ -274,7 +281,7 @@ void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* 
         else id = it->second;
         vid[e * NV + v] = id;
       }
-    if (D == 3 && ids.size() >= ((size_t)1 << 21)) throw Unsupported("more than 2^21 mesh vertices");
+    if (ids.size() >= ((size_t)1 << 31)) throw Unsupported("more than 2^31 mesh vertices");
   }
   std::vector<uint8_t> bcf(K * c.Nf, 0);
   if (m->node_bc)
@@ -347,6 +354,7 @@ StageArgs base_args(Ctx& c) {
   StageArgs a{};
   a.nq = c.variant == kVariantLds ? 4 : c.nq;
   a.tile_order = c.tile_order;
+  a.addr64 = c.addr64 ? 1 : 0;
   a.K = c.K;
   a.Kpad = c.Kpad;
   a.PS = c.PS;
@@ -517,19 +525,19 @@ std::vector<int64_t> locate_points(const RefElem& R, const double* node_xyz, int
 void install_receivers(Ctx& c, const std::vector<int64_t>& elem, const std::vector<double>& w, int cap) {
   DGK_HIP(hipStreamSynchronize(c.stream));
   c.free_receivers();
-  if (c.graph) { DGK_HIP(hipGraphExecDestroy(c.graph)); c.graph = nullptr; }
+  c.drop_graphs();
   const int n = (int)elem.size();
   if (!n) return;
   if (cap < 1) throw InvalidArg("receiver capacity must be positive");
-  std::vector<int> base(n);
-  for (int i = 0; i < n; ++i) base[i] = (int)(elem[i] * c.Np);
-  DGK_HIP(hipMalloc(&c.rec.base, n * sizeof(int)));
+  std::vector<int64_t> base(n);
+  for (int i = 0; i < n; ++i) base[i] = elem[i] * c.Np;
+  DGK_HIP(hipMalloc(&c.rec.base, n * sizeof(int64_t)));
   DGK_HIP(hipMalloc(&c.rec.w, (size_t)n * c.Np * c.tsize));
   DGK_HIP(hipMalloc(&c.rec.out, (size_t)n * cap * c.tsize));
   DGK_HIP(hipMalloc(&c.rec.counter, sizeof(int)));
   c.rec.n = n;
   c.rec.cap = cap;
-  DGK_HIP(hipMemcpy(c.rec.base, base.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  DGK_HIP(hipMemcpy(c.rec.base, base.data(), n * sizeof(int64_t), hipMemcpyHostToDevice));
   if (c.precision == DGK_FP64) {
     DGK_HIP(hipMemcpy(c.rec.w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
   } else {
@@ -570,44 +578,65 @@ bool sources_active(Ctx& c, double t) {
   return false;
 }
 
+// The graph of `steps` RK4 steps (each preceded by the timed source overwrite when src), captured
+// once per (steps, src) and time step.
+hipGraphExec_t rk4_graph(Ctx& c, int steps, double dt, bool src) {
+  if (c.graph_dt != dt) c.drop_graphs();
+  c.graph_dt = dt;
+  auto it = c.graphs.find({steps, src});
+  if (it != c.graphs.end()) return it->second;
+  hipGraph_t g;
+  DGK_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  try {
+    for (int k = 0; k < steps; ++k) {
+      if (src) {
+        for (auto& s : c.sources) launch_source_timed(c, s, c.stream);
+        launch_tick(c, c.stream);
+      }
+      rk4_step(c, dt, c.stream);
+    }
+  } catch (...) {
+    hipGraph_t junk = nullptr;
+    if (hipStreamEndCapture(c.stream, &junk) == hipSuccess && junk) (void)hipGraphDestroy(junk);
+    throw;
+  }
+  DGK_HIP(hipStreamEndCapture(c.stream, &g));
+  hipGraphExec_t ex = nullptr;
+  hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  DGK_HIP(e);
+  c.graphs[{steps, src}] = ex;
+  return ex;
+}
+
+// RK4 steps replayed from captured graphs of up to kChunk steps (the remainder gets a graph of its
+// own length). With sources, the steps' start times go to the device first and the captured
+// source kernels read the current step's time there, so source steps replay from graphs too.
 void rk4_impl(Ctx& c, int nsteps, double dt, const double* ts) {
   if (nsteps < 0) throw InvalidArg("nsteps < 0");
   constexpr int kChunk = 25;
-  int i = 0;
-  while (i < nsteps) {
-    double t = ts ? ts[i] : 0.0;
-    bool src = !c.sources.empty() && ts && sources_active(c, t);
-    if (!src && nsteps - i >= kChunk) {
-      // graph of kChunk steps (no sources active in it)
-      bool any = false;
-      for (int k = i; k < i + kChunk && ts && !c.sources.empty(); ++k) any |= sources_active(c, ts[k]);
-      if (!any) {
-        if (!c.graph || c.graph_steps != kChunk || c.graph_dt != dt) {
-          if (c.graph) { DGK_HIP(hipGraphExecDestroy(c.graph)); c.graph = nullptr; }
-          hipGraph_t g;
-          DGK_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-          try {
-            for (int k = 0; k < kChunk; ++k) rk4_step(c, dt, c.stream);
-          } catch (...) {
-            hipGraph_t junk;
-            (void)hipStreamEndCapture(c.stream, &junk);
-            throw;
-          }
-          DGK_HIP(hipStreamEndCapture(c.stream, &g));
-          hipError_t e = hipGraphInstantiate(&c.graph, g, nullptr, nullptr, 0);
-          (void)hipGraphDestroy(g);
-          DGK_HIP(e);
-          c.graph_steps = kChunk;
-          c.graph_dt = dt;
-        }
-        DGK_HIP(hipGraphLaunch(c.graph, c.stream));
-        i += kChunk;
-        continue;
-      }
+  bool src = false;
+  if (ts && !c.sources.empty())
+    for (int i = 0; i < nsteps && !src; ++i) src = sources_active(c, ts[i]);
+  if (src) {
+    if (nsteps > c.tcap) {  // the captures hold the old address
+      DGK_HIP(hipStreamSynchronize(c.stream));
+      c.drop_graphs();
+      if (c.tdev) DGK_HIP(hipFree(c.tdev));
+      c.tdev = nullptr;
+      c.tcap = 0;
+      DGK_HIP(hipMalloc(&c.tdev, (size_t)std::max(nsteps, 1024) * sizeof(double)));
+      c.tcap = std::max(nsteps, 1024);
     }
-    if (ts) apply_sources(c, t);
-    rk4_step(c, dt, c.stream);
-    ++i;
+    if (!c.tstep) c.tstep = (int*)c.alloc(sizeof(int));
+    DGK_HIP(hipMemcpyAsync(c.tdev, ts, (size_t)nsteps * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    DGK_HIP(hipMemsetAsync(c.tstep, 0, sizeof(int), c.stream));
+    DGK_HIP(hipStreamSynchronize(c.stream));  // ts is the caller's pageable buffer
+  }
+  for (int i = 0; i < nsteps;) {
+    const int n = std::min(kChunk, nsteps - i);
+    DGK_HIP(hipGraphLaunch(rk4_graph(c, n, dt, src), c.stream));
+    i += n;
   }
 }
 
@@ -621,7 +650,7 @@ void euler_impl(Ctx& c, int nsteps, double dt, const double* ts) {
     a.Vout = c.V0;
     launch_stage(c, a, c.stream);
     std::swap(c.U, c.V0);  // graphs capture buffer addresses; drop any cached one
-    if (c.graph) { DGK_HIP(hipGraphExecDestroy(c.graph)); c.graph = nullptr; }
+    c.drop_graphs();
     launch_record(c, c.stream);
   }
 }
@@ -724,10 +753,7 @@ dgk_status dgk_set_state(dgk_ctx* c, const double* u) {
   if (!c || !u) return fail(c, DGK_ERR_INVALID, "NULL argument");
   return guarded(c, [&] {
     const int nq = live_planes(c->dim, u, c->K * c->Np);
-    if (nq != c->nq && c->graph) {  // captured launches carry the plane count
-      DGK_HIP(hipGraphExecDestroy(c->graph));
-      c->graph = nullptr;
-    }
+    if (nq != c->nq) c->drop_graphs();  // captured launches carry the plane count
     c->nq = nq;
     if (c->es > 1) {
       c->es = nq;  // interleaved buffers hold exactly the live fields; the upload below relays U
@@ -755,24 +781,25 @@ dgk_status dgk_set_sources(dgk_ctx* c, int n, const double* src, const double* n
   if (!c || n < 0 || (n > 0 && (!src || !node_xyz))) return fail(c, DGK_ERR_INVALID, "bad source arguments");
   return guarded(c, [&] {
     DGK_HIP(hipStreamSynchronize(c->stream));
+    c->drop_graphs();  // source steps are captured with the source list
     for (auto& s : c->sources)
       if (s.idx) DGK_HIP(hipFree(s.idx));
     c->sources.clear();
     const int64_t KN = c->K * c->Np;
     for (int s = 0; s < n; ++s) {
       const double* S = src + 9 * s;
-      std::vector<int> idx;
+      std::vector<int64_t> idx;
       for (int64_t i = 0; i < KN; ++i) {
         const double* X = node_xyz + 3 * i;
         if (std::pow(X[0] - S[1], 2) + std::pow(X[1] - S[2], 2) + std::pow(X[2] - S[3], 2) < std::pow(S[4], 2))
-          idx.push_back((int)i);  // pressure is equation 0: U[0*KN + i]
+          idx.push_back(i);  // pressure is equation 0: U[0*KN + i]
       }
       Ctx::Src d;
       d.amp = S[5]; d.freq = S[6]; d.phase = S[7]; d.duration = S[8];
       d.n = (int64_t)idx.size();
       if (d.n) {
-        DGK_HIP(hipMalloc(&d.idx, idx.size() * sizeof(int)));
-        DGK_HIP(hipMemcpy(d.idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+        DGK_HIP(hipMalloc(&d.idx, idx.size() * sizeof(int64_t)));
+        DGK_HIP(hipMemcpy(d.idx, idx.data(), idx.size() * sizeof(int64_t), hipMemcpyHostToDevice));
       }
       c->sources.push_back(d);
     }
@@ -809,7 +836,7 @@ dgk_status dgk_set_launch(dgk_ctx* c, int grid_pct, int tile_order) {
     DGK_HIP(hipStreamSynchronize(c->stream));
     c->grid_pct = grid_pct;
     c->tile_order = tile_order;
-    if (c->graph) { DGK_HIP(hipGraphExecDestroy(c->graph)); c->graph = nullptr; }
+    c->drop_graphs();
   });
 }
 
@@ -955,11 +982,15 @@ struct dgk_batch {
   hipStream_t main = nullptr;
   hipEvent_t fork = nullptr;
   std::vector<hipEvent_t> joins;
-  hipGraphExec_t graph = nullptr;
-  int graph_steps = 0, graph_schedule = -1;
+  // captured step sequences by (steps, schedule)
+  std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+  void drop_graphs() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    graphs.clear();
+  }
   std::string err;
   ~dgk_batch() {
-    if (graph) (void)hipGraphExecDestroy(graph);
+    drop_graphs();
     for (auto& g : groups) delete g.ctx;
     for (auto e : joins) (void)hipEventDestroy(e);
     if (fork) (void)hipEventDestroy(fork);
@@ -991,11 +1022,10 @@ void batch_create_impl(dgk_batch& B, int n, const dgk_mesh* meshes, const dgk_ph
   B.K.resize(n);
   B.nq.assign(n, 0);
   // Group key: dim, order, physics. A group closes when adding a member would break the per-context
-  // limits (4 GiB per state buffer, 2^21 vertices in 3D); the next member of that key opens a new one.
+  // limits (2^31 element faces, 2^31 vertices); the next member of that key opens a new one.
   std::vector<std::vector<int>> members;
   std::vector<std::array<double, 7>> keys;
   std::vector<int64_t> gK, gV;
-  const int64_t tsz = (opts && opts->precision == DGK_FP32) ? 4 : 8;
   for (int i = 0; i < n; ++i) {
     const dgk_mesh& m = meshes[i];
     if (!(dt[i] > 0)) throw InvalidArg("batch time steps must be positive");
@@ -1013,8 +1043,8 @@ void batch_create_impl(dgk_batch& B, int n, const dgk_mesh* meshes, const dgk_ph
                             phys[i].v0[0], phys[i].v0[1], phys[i].v0[2]};
     size_t gi = 0;
     while (gi < keys.size() &&
-           (keys[gi] != k || 4 * (gK[gi] + m.num_elements + kTilePad) * Np * tsz >= ((int64_t)1 << 32) ||
-            (m.dim == 3 && gV[gi] + nv >= ((int64_t)1 << 21))))
+           (keys[gi] != k || (gK[gi] + m.num_elements + kTilePad) * (m.dim + 1) >= ((int64_t)1 << 31) ||
+            gV[gi] + nv >= ((int64_t)1 << 31)))
       ++gi;
     if (gi == keys.size()) {
       keys.push_back(k);
@@ -1087,11 +1117,11 @@ void batch_enqueue(dgk_batch& B, int nsteps, int schedule) {
   }
 }
 
-// Capture (once) the graph of `chunk` steps of every group under `schedule`.
-void batch_graph(dgk_batch& B, int chunk, int schedule) {
+// The graph of `chunk` steps of every group under `schedule`, captured once.
+hipGraphExec_t batch_graph(dgk_batch& B, int chunk, int schedule) {
   using namespace dgk;
-  if (B.graph && B.graph_steps == chunk && B.graph_schedule == schedule) return;
-  if (B.graph) { DGK_HIP(hipGraphExecDestroy(B.graph)); B.graph = nullptr; }
+  auto it = B.graphs.find({chunk, schedule});
+  if (it != B.graphs.end()) return it->second;
   hipGraph_t gr;
   DGK_HIP(hipStreamBeginCapture(B.main, hipStreamCaptureModeGlobal));
   try {
@@ -1102,14 +1132,17 @@ void batch_graph(dgk_batch& B, int chunk, int schedule) {
     throw;
   }
   DGK_HIP(hipStreamEndCapture(B.main, &gr));
-  hipError_t e = hipGraphInstantiate(&B.graph, gr, nullptr, nullptr, 0);
+  hipGraphExec_t ex = nullptr;
+  hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
   (void)hipGraphDestroy(gr);
   DGK_HIP(e);
-  B.graph_steps = chunk;
-  B.graph_schedule = schedule;
+  B.graphs[{chunk, schedule}] = ex;
+  return ex;
 }
 
-constexpr int kMaxGraphSteps = 1000;
+// Steps per captured graph: a 256-simulation ensemble has about 32 stage launches per step, so a
+// graph holds a few thousand kernel nodes and is replayed nsteps/100 times.
+constexpr int kMaxGraphSteps = 100;
 
 void batch_check(int nsteps, int schedule) {
   if (nsteps < 0 || schedule < 0 || schedule > 1) throw dgk::InvalidArg("bad batch step arguments");
@@ -1120,16 +1153,13 @@ void batch_rk4_impl(dgk_batch& B, int nsteps, int schedule) {
   batch_check(nsteps, schedule);
   for (auto& G : B.groups)  // the batch stream runs after anything queued on the contexts
     DGK_HIP(hipStreamSynchronize(G.ctx->stream));
-  // Up to kMaxGraphSteps steps of every group are captured as one graph and replayed; a
-  // capture is reused while the step count and schedule stay the same.
-  const int chunk = std::min(nsteps, kMaxGraphSteps);
-  int done = 0;
-  while (chunk > 0 && nsteps - done >= chunk) {
-    batch_graph(B, chunk, schedule);
-    DGK_HIP(hipGraphLaunch(B.graph, B.main));
+  // Up to kMaxGraphSteps steps of every group are captured as one graph and replayed, the
+  // remainder as a graph of its own length; captures are kept per (steps, schedule).
+  for (int done = 0; done < nsteps;) {
+    const int chunk = std::min(nsteps - done, kMaxGraphSteps);
+    DGK_HIP(hipGraphLaunch(batch_graph(B, chunk, schedule), B.main));
     done += chunk;
   }
-  if (done < nsteps) batch_enqueue(B, nsteps - done, schedule);
 }
 
 }  // namespace
@@ -1174,9 +1204,8 @@ dgk_status batch_state_io(dgk_batch* b, int sim, double* u, bool to_device) {
       b->nq[sim] = dgk::live_planes(c.dim, u, n);
       int nq = c.dim + 1;
       for (int i : b->groups[w.first].sims) nq = std::max(nq, b->nq[i]);
-      if (nq != c.nq && b->graph) {
-        DGK_HIP(hipGraphExecDestroy(b->graph));
-        b->graph = nullptr;
+      if (nq != c.nq && !b->graphs.empty()) {
+        b->drop_graphs();
       }
       if (nq != c.nq && c.es > 1) {
         // interleaved buffers hold exactly the live fields: relay the whole group's state
@@ -1246,7 +1275,7 @@ dgk_status dgk_batch_set_receivers(dgk_batch* b, int sim, int n, const double* x
       allw.insert(allw.end(), G.rw[k].begin(), G.rw[k].end());
     }
     dgk::install_receivers(c, all, allw, G.rcap);
-    if (b->graph) { DGK_HIP(hipGraphExecDestroy(b->graph)); b->graph = nullptr; }
+    b->drop_graphs();
   });
 }
 
@@ -1282,7 +1311,7 @@ dgk_status dgk_batch_set_launch(dgk_batch* b, int group, int grid_pct, int tile_
     dgk::Ctx& c = *b->groups[group].ctx;
     c.grid_pct = grid_pct;
     c.tile_order = tile_order;
-    if (b->graph) { DGK_HIP(hipGraphExecDestroy(b->graph)); b->graph = nullptr; }
+    b->drop_graphs();
   });
 }
 
@@ -1291,6 +1320,7 @@ dgk_status dgk_batch_prepare(dgk_batch* b, int nsteps, int schedule) {
   return bguard(b, [&] {
     batch_check(nsteps, schedule);
     if (nsteps > 0) batch_graph(*b, std::min(nsteps, kMaxGraphSteps), schedule);
+    if (nsteps > kMaxGraphSteps && nsteps % kMaxGraphSteps) batch_graph(*b, nsteps % kMaxGraphSteps, schedule);
   });
 }
 

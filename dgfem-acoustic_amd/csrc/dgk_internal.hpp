@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -77,6 +78,7 @@ struct StageArgs {
   int tiles_per_xcd;
   int64_t tiles;
   int tile_order;  // 0: XCD-aware (XCD x sweeps a contiguous slab of tiles), 1: round-robin
+  int addr64;      // 1: 64-bit per-tile bases even when every array fits 32-bit offsets
 };
 
 struct Ctx {
@@ -105,6 +107,7 @@ struct Ctx {
   // launch shape (tunable): persistent grid as a percentage of the occupancy-derived grid, and
   // the tile order
   int grid_pct = 100, tile_order = 0;
+  bool addr64 = false;  // force the 64-bit addressing form of the stage kernel
   int32_t *nbr = nullptr, *code = nullptr, *nface = nullptr, *perm = nullptr;
   double* stage_d = nullptr;  // FP64 staging for state transfer in FP32 mode
   int* flag = nullptr;
@@ -114,25 +117,30 @@ struct Ctx {
   // sources
   struct Src {
     double amp, freq, phase, duration;
-    int* idx = nullptr;
+    int64_t* idx = nullptr;  // node offsets e*Np + node of the covered element nodes
     int64_t n = 0;
   };
   std::vector<Src> sources;
   // receivers: pressure interpolated at points (element Lagrange basis), recorded after every step
   struct Rec {
     int n = 0, cap = 0;
-    int* base = nullptr;     // [n] offset e*Np of the receiver's element in the pressure plane
+    int64_t* base = nullptr;  // [n] offset e*Np of the receiver's element in the pressure plane
     void* w = nullptr;       // [n][Np] basis weights in the state type
     void* out = nullptr;     // [cap][n] recorded pressure in the state type
     int* counter = nullptr;  // rows recorded so far (device)
   } rec;
   void free_receivers();
   int64_t device_bytes = 0;
-  // graph cache
-  hipGraphExec_t graph = nullptr;
-  int graph_steps = 0;
+  // Captured RK4 step sequences, by (steps, sources launched) for time step graph_dt. Captures
+  // hold buffer addresses and launch settings: drop_graphs() whenever those change.
+  std::map<std::pair<int, bool>, hipGraphExec_t> graphs;
   double graph_dt = 0;
-  int graph_euler = 0;
+  void drop_graphs();
+  // Start times of the steps of the current dgk_rk4 call on the device (sources read
+  // tdev[*tstep]; *tstep advances after every step)
+  double* tdev = nullptr;
+  int tcap = 0;
+  int* tstep = nullptr;
   std::string err;
   std::vector<void*> allocs;
 
@@ -154,6 +162,9 @@ void scale_geometry(Ctx& c, int64_t e0, int64_t n, double s);
 // stage.hip
 void launch_stage(Ctx& c, StageArgs& a, hipStream_t s);
 void launch_source(Ctx& c, const Ctx::Src& s, double value, hipStream_t st);
+// hard source overwrite at the step whose start time is c.tdev[*c.tstep] (when t < duration)
+void launch_source_timed(Ctx& c, const Ctx::Src& s, hipStream_t st);
+void launch_tick(Ctx& c, hipStream_t st);  // ++*c.tstep
 void launch_record(Ctx& c, hipStream_t st);  // no-op without receivers
 bool check_finite(Ctx& c);
 // FP64 field planes on the device (plane stride PS; element nodes [off, off+n) of each plane) -> a

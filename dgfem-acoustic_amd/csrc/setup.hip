@@ -1,12 +1,14 @@
 // Device-side setup: face connectivity built on the GPU and per-element affine geometry.
 //
 // Replaces the reference's quadratic face search and neighbour bookkeeping
-// (src/Mesh.cpp:107-128, 235-314, 679-713) with: one 64-bit key per element face
-// (sorted corner-vertex ids), a device radix sort of (key, element-face) pairs, a pairing
-// pass over equal neighbouring keys and a finalisation pass that records the neighbour
+// (src/Mesh.cpp:107-128, 235-314, 679-713) with: one key per element face (its sorted
+// corner-vertex ids: 32 bits in 1D, 64 in 2D, 96 in 3D as a 32-bit high and a 64-bit low part),
+// a device radix sort of (key, element-face) pairs (3D: two stable passes, low part first), a
+// pairing pass over equal neighbouring keys and a finalisation pass that records the neighbour
 // element, its local face, the vertex permutation between the two faces and the interior
 // penalty sign. Radix sort is stable, so for a shared face the lower element index comes
 // first, which is the reference's "up" side (src/Mesh.cpp:236-246).
+#include <algorithm>
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -20,7 +22,7 @@ namespace {
 __constant__ int c_fv[4][3];
 
 __global__ void k_face_keys(int64_t K, int D, int NF, const int32_t* __restrict__ vid, uint64_t* __restrict__ keys,
-                            int32_t* __restrict__ vals) {
+                            uint32_t* __restrict__ hi, int32_t* __restrict__ vals) {
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= K * NF) return;
   int64_t e = t / NF;
@@ -33,18 +35,29 @@ __global__ void k_face_keys(int64_t K, int D, int NF, const int32_t* __restrict_
     if (a[1] > a[2]) { uint64_t x = a[1]; a[1] = a[2]; a[2] = x; }
     if (a[0] > a[1]) { uint64_t x = a[0]; a[0] = a[1]; a[1] = x; }
   }
-  uint64_t key = D == 3 ? (a[0] << 42) | (a[1] << 21) | a[2] : D == 2 ? (a[0] << 32) | a[1] : a[0];
-  keys[t] = key;
+  // 3D: keys holds the low part (a1, a2), hi the smallest id a0
+  keys[t] = D == 3 ? (a[1] << 32) | a[2] : D == 2 ? (a[0] << 32) | a[1] : a[0];
+  if (D == 3) hi[t] = (uint32_t)a[0];
   vals[t] = (int32_t)t;
 }
 
-__global__ void k_face_pair(int64_t n, const uint64_t* __restrict__ keys, const int32_t* __restrict__ vals,
-                            int32_t* __restrict__ partner, int* __restrict__ err) {
+// high key parts in the order of the first (low-part) sort
+__global__ void k_gather_hi(int64_t n, const uint32_t* __restrict__ hi, const int32_t* __restrict__ vals,
+                            uint32_t* __restrict__ out) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = hi[vals[i]];
+}
+
+// vals: element faces in key order; keys (and hi, 3D) indexed by element face
+__global__ void k_face_pair(int64_t n, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ hi,
+                            const int32_t* __restrict__ vals, int32_t* __restrict__ partner, int* __restrict__ err) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint64_t k = keys[i];
-  bool prev = i > 0 && keys[i - 1] == k;
-  bool next = i + 1 < n && keys[i + 1] == k;
+  auto same = [&](int64_t a, int64_t b) {
+    return keys[vals[a]] == keys[vals[b]] && (!hi || hi[vals[a]] == hi[vals[b]]);
+  };
+  bool prev = i > 0 && same(i - 1, i);
+  bool next = i + 1 < n && same(i + 1, i);
   if (prev && next) atomicOr(err, 1);  // a face shared by three or more elements
   partner[vals[i]] = prev ? vals[i - 1] : next ? vals[i + 1] : -1;
 }
@@ -200,12 +213,13 @@ void build_connectivity(Ctx& c, const double* vxyz_dev, const int32_t* vid_host,
   hipStream_t s = c.stream;
   int32_t *vid = nullptr, *vals = nullptr, *vals2 = nullptr, *partner = nullptr;
   uint64_t *keys = nullptr, *keys2 = nullptr;
+  uint32_t *hi = nullptr, *hi2 = nullptr, *hi3 = nullptr;
   uint8_t* bcf = nullptr;
   int* err = nullptr;
   void* tmp = nullptr;
   auto cleanup = [&]() {
-    for (void* p : {(void*)vid, (void*)vals, (void*)vals2, (void*)partner, (void*)keys, (void*)keys2, (void*)bcf,
-                    (void*)err, tmp})
+    for (void* p : {(void*)vid, (void*)vals, (void*)vals2, (void*)partner, (void*)keys, (void*)keys2, (void*)hi,
+                    (void*)hi2, (void*)hi3, (void*)bcf, (void*)err, tmp})
       if (p) (void)hipFree(p);
   };
   try {
@@ -215,6 +229,11 @@ void build_connectivity(Ctx& c, const double* vxyz_dev, const int32_t* vid_host,
     DGK_HIP(hipMalloc(&partner, sizeof(int32_t) * n));
     DGK_HIP(hipMalloc(&keys, sizeof(uint64_t) * n));
     DGK_HIP(hipMalloc(&keys2, sizeof(uint64_t) * n));
+    if (D == 3) {
+      DGK_HIP(hipMalloc(&hi, sizeof(uint32_t) * n));
+      DGK_HIP(hipMalloc(&hi2, sizeof(uint32_t) * n));
+      DGK_HIP(hipMalloc(&hi3, sizeof(uint32_t) * n));
+    }
     DGK_HIP(hipMalloc(&bcf, n));
     DGK_HIP(hipMalloc(&err, sizeof(int)));
     DGK_HIP(hipMemcpyAsync(vid, vid_host, sizeof(int32_t) * c.K * (D + 1), hipMemcpyHostToDevice, s));
@@ -222,14 +241,28 @@ void build_connectivity(Ctx& c, const double* vxyz_dev, const int32_t* vid_host,
     DGK_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
     int bs = 256;
     int64_t nb = (n + bs - 1) / bs;
-    hipLaunchKernelGGL(k_face_keys, dim3((unsigned)nb), dim3(bs), 0, s, c.K, D, NF, vid, keys, vals);
+    hipLaunchKernelGGL(k_face_keys, dim3((unsigned)nb), dim3(bs), 0, s, c.K, D, NF, vid, keys, hi, vals);
     DGK_HIP(hipGetLastError());
-    size_t tmp_bytes = 0;
-    int end_bit = D == 3 ? 63 : D == 2 ? 64 : 32;
+    size_t tmp_bytes = 0, tmp_hi = 0;
+    const int end_bit = D == 1 ? 32 : 64;
     DGK_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys, keys2, vals, vals2, (size_t)n, 0, end_bit, s));
+    if (D == 3) {
+      DGK_HIP(rocprim::radix_sort_pairs(nullptr, tmp_hi, hi2, hi3, vals2, vals, (size_t)n, 0, 32, s));
+      tmp_bytes = std::max(tmp_bytes, tmp_hi);
+    }
     DGK_HIP(hipMalloc(&tmp, tmp_bytes));
     DGK_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, keys, keys2, vals, vals2, (size_t)n, 0, end_bit, s));
-    hipLaunchKernelGGL(k_face_pair, dim3((unsigned)nb), dim3(bs), 0, s, n, keys2, vals2, partner, err);
+    const int32_t* order = vals2;
+    if (D == 3) {
+      // second, stable pass on the high part: faces with equal (a0, a1, a2) end up adjacent, in
+      // element-face order
+      hipLaunchKernelGGL(k_gather_hi, dim3((unsigned)nb), dim3(bs), 0, s, n, hi, vals2, hi2);
+      DGK_HIP(hipGetLastError());
+      DGK_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, hi2, hi3, vals2, vals, (size_t)n, 0, 32, s));
+      order = vals;
+    }
+    hipLaunchKernelGGL(k_face_pair, dim3((unsigned)nb), dim3(bs), 0, s, n, keys, (const uint32_t*)hi, order, partner,
+                       err);
     DGK_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_face_finalize, dim3((unsigned)nb), dim3(bs), 0, s, c.K, D, NF, c.nperm, vid, vxyz_dev,
                        partner, bcf, c.sign_mode == DGK_SIGN_REFERENCE ? 1 : 0, (int)c.fc, sigma, c.nbr, c.code,

@@ -295,7 +295,9 @@ __device__ __forceinline__ int d_row<double>(int lane, int j) { return (lane >> 
 // are still formed from U alone, which costs only a non-accumulating 2^-24 perturbation.
 // Element access through a uniform base pointer plus a 32-bit byte offset, which maps to the
 // global_load/store form with an SGPR base and a 32-bit VGPR offset (no 64-bit address arithmetic
-// per access). Offsets stay below 2^32: a context holds 4*PS*sizeof(T) < 2^32 bytes per buffer.
+// per access). Contexts whose buffers stay below 4 GiB use the buffers' own bases; larger ones
+// (the BIG instantiation of the stage kernel) form a 64-bit base per tile on the scalar unit and
+// the neighbour element's base per gather, with 32-bit offsets inside those.
 template <typename P>
 __device__ __forceinline__ P* byte_at(P* base, uint32_t off) {
   return (P*)((char*)base + off);
@@ -315,13 +317,27 @@ __device__ __forceinline__ uint32_t f32_to_bf16_bits(float f) {
 // tiles) so that two waves per SIMD share one operator copy in LDS: FP32 order 5 (64.5 KB
 // operator) and FP64 order 4 (48.4 KB operator, 50 KB of tile buffers per 16 elements) have room
 // for one workgroup per CU; FP32 order 4 (24 KB operator) is 2-4% faster than with two 4-wave
-// workgroups. FP64 order 5 reads its operator from global memory and keeps 4 waves.
+// workgroups.
+// FP64 order 5 (129 KB operator) splits rows instead of elements: a 4-wave workgroup owns a tile of
+// 4 elements (21 KB of tile buffers beside the operator in LDS) and wave w computes row block w of
+// all 16 columns, so the operator never leaves LDS; every wave builds the same B operand.
 template <typename T, int D, int P>
 struct MfmaLaunch {
-  static constexpr int WAVES = (D == 3 && (P == 5 || P == 4) && !(sizeof(T) == 8 && P == 5)) ? 8 : 4;
-  static constexpr int KB = 64 * WAVES;  // threads per workgroup
-  static constexpr int TE = 4 * WAVES;   // elements per tile
+  static constexpr bool ROWSPLIT = sizeof(T) == 8 && D == 3 && P == 5;
+  static constexpr int WAVES = (D == 3 && (P == 5 || P == 4) && !ROWSPLIT) ? 8 : 4;
+  static constexpr int KB = 64 * WAVES;              // threads per workgroup
+  static constexpr int TE = ROWSPLIT ? 4 : 4 * WAVES;  // elements per tile
+  static constexpr int RG = WAVES / (TE / 4);        // row groups: waves sharing one element group
 };
+
+// Pad (in values) of the LDS element stride NP*ES: FP32 to 4 or 12 mod 16 words, FP64 to 32 mod 64
+// words (the 4 elements of a wave then use different banks); none for the 2- and 3-field layouts
+template <typename T, int ES>
+constexpr int lds_pad(int np) {
+  if (ES != 4) return 0;
+  if (sizeof(T) == 4) return np % 4 == 0 || np % 4 == 2 ? 4 : 0;
+  return ((32 - 8 * (np % 8)) % 64 + 64) % 64 / 2;
+}
 
 // ES consecutive field values of one node (ES = 4: one vector read), zero beyond ES
 template <int ES, typename T>
@@ -337,7 +353,7 @@ __device__ __forceinline__ void node_fields(const T* p, T f[4]) {
   }
 }
 
-template <typename T, int D, int P, bool LO, bool CS, int ES>
+template <typename T, int D, int P, bool LO, bool CS, int ES, bool BIG>
 __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageArgs a) {
   static_assert(!CS || sizeof(T) == 4, "the compensated state is an FP32 form");
   using S = Sz<D, P>;
@@ -351,6 +367,13 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   // distinct values (row l%4 at lane group l/16; lanes 4 apart in l/4 read the same value)
   constexpr int NAV = (MBF * 64 + NT4 * 16) * KS;    // operator values
   constexpr int TE = MfmaLaunch<T, D, P>::TE;
+  // row groups: RG waves share one group of 4 elements, wave rg computing row blocks
+  // [rg*MBW, (rg+1)*MBW); with a single row block per wave the k-steps alternate between two
+  // partial accumulators (summed before the write-out) so that consecutive MFMAs are independent
+  constexpr int RG = MfmaLaunch<T, D, P>::RG;
+  static_assert(RG == 1 || (NT4 == 0 && !LO && MBF % RG == 0), "row split: whole 16-row blocks only");
+  constexpr int MBW = MBF / RG;
+  constexpr int NCH = (RG > 1 && MBW == 1) ? 2 : 1;
   constexpr int VW = 16 / sizeof(T);                        // values per 16-byte vector
   typedef T VT __attribute__((ext_vector_type(VW)));
   constexpr int PV = TE * NP / VW;                          // vectors per field of a tile (ES*PV in all)
@@ -363,18 +386,26 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   constexpr int NGEO = (TE * 9 + kBlock - 1) / kBlock;      // geometry values per thread
   constexpr int NFC = (TE * NF + kBlock - 1) / kBlock;      // element faces per thread
   typedef typename Mfma<T>::V V;
-  __shared__ __attribute__((aligned(32))) T sU[4][TE * NP];
+  // LDS element stride LES of the state tile and of the results: NP*ES values plus a pad (4-field
+  // layouts only, whose element boundaries fall on whole 16-byte vectors) chosen so that the four
+  // elements of a wave fall in different LDS banks. A volume k-step's lanes read node 4m+g of their
+  // element (16 or 32 bytes) and the result write-out puts row 4g+j (FP32) or g+4j (FP64) of 4
+  // elements x 4 fields: FP32 wants the stride in 4-byte words = 4 mod 16 (or 12), FP64 32 mod 64.
+  constexpr int PADV = lds_pad<T, ES>(NP);
+  constexpr int LES = NP * ES + PADV;
+  __shared__ __attribute__((aligned(32))) T sU[TE * LES];
   // face fluxes [element][field][RB], then the results in sU's layout. A face k-step's 64 lanes read
   // words c*RB + g + const (c < 16, g < 4): for 4-byte values that is 64 distinct LDS banks whenever
   // RB is 4 times an odd number (3D orders 4 and 5: 60, 84)
-  __shared__ __attribute__((aligned(16))) T sB[4 * TE * RB];
+  constexpr int NSB = 4 * TE * RB > TE * LES ? 4 * TE * RB : TE * LES;
+  __shared__ __attribute__((aligned(16))) T sB[NSB];
   __shared__ T sN[TE * NF][4];
   __shared__ T sG[TE][9];
   __shared__ int sVmap[NF * NPERM * NFP];
   __shared__ int sFmask[NF * NFP];
   // Operator fragments are staged in LDS whenever the whole workgroup footprint fits the 160 KiB
   // of a gfx950 CU (fewer resident workgroups, but no per-k-step global loads).
-  constexpr size_t kLdsRest = sizeof(T) * (4 * TE * NP + 4 * TE * RB + TE * NF * 4 + TE * 9) +
+  constexpr size_t kLdsRest = sizeof(T) * (TE * LES + NSB + TE * NF * 4 + TE * 9) +
                               sizeof(int) * (NF * NPERM * NFP + NF * NFP) + (LO ? 16 * MB * NLO * 64 : 0);
   constexpr bool kALds = NAV * sizeof(T) + kLdsRest <= 160 * 1024 - 512;
   __shared__ T sA[kALds ? NAV : 1];
@@ -382,7 +413,9 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   T* const sR = sB;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int g = lane >> 4, col = lane & 15, el = wave * 4 + (col >> 2), q = col & 3;
+  // wave = rg * (TE/4) + element group; lane group g, column col = 4 * (element in group) + field q
+  const int rg = RG > 1 ? wave / (TE / 4) : 0;
+  const int g = lane >> 4, col = lane & 15, el = (wave % (TE / 4)) * 4 + (col >> 2), q = col & 3;
   // 32-bit byte offsets: a buffer holds 4*PS*sizeof(T) < 2^32 bytes (checked at context creation)
   const T* __restrict__ Vin = (const T*)a.Vin;
   T* __restrict__ Vout = (T*)a.Vout;
@@ -401,7 +434,8 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   for (int i = tid; i < NF * NFP; i += kBlock) sFmask[i] = a.fmask[i];
   if (kALds)
     for (int i = tid; i < NAV; i += kBlock) sA[i] = Af[i];
-  const T* __restrict__ pA = (kALds ? (const T*)sA : Af) + lane;
+  static_assert(RG == 1 || kALds, "row split keeps the operator in LDS");
+  const T* __restrict__ pA = (kALds ? (const T*)sA : Af) + lane + rg * (MBW * KS * 64);
   const T* __restrict__ pT = (kALds ? (const T*)sA : Af) + MBF * KS * 64 + 4 * (lane >> 4) + (lane & 3);
   if (LO)
     for (int i = tid; i < MB * NLO * 64; i += kBlock) sAlo[i] = ((const bf16x8*)a.Alo)[i];
@@ -415,7 +449,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   for (int t = 0; t < NE; ++t) {
     const int v = tid + t * kBlock;
     voff[t] = (uint32_t)(v * VW);
-    loff[t] = v * VW;
+    loff[t] = v * VW + (v * VW / (NP * ES)) * PADV;
   }
 
   // Software pipeline: while tile t is processed, tile t+1's state, geometry and face geometry,
@@ -425,27 +459,34 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   T pgeo[NGEO], pfg[NFC][4];
   int2 inc[NI], cnc[NI];
   T iqp[NI][4];
-  auto prefetch = [&](int e0) {
+  // value `rel` of an array of `per` values per element, at the tile starting at element e0 (32-bit
+  // arithmetic unless BIG)
+  auto tile_at = [&](auto* base, int64_t e0, uint32_t per, uint32_t rel) {
+    using E = std::remove_cv_t<std::remove_pointer_t<decltype(base)>>;
+    if constexpr (BIG) return byte_at(base + e0 * per, rel * (uint32_t)sizeof(E));
+    else return byte_at(base, ((uint32_t)e0 * per + rel) * (uint32_t)sizeof(E));
+  };
+  auto prefetch = [&](int64_t e0) {
 #pragma unroll
     for (int t = 0; t < NE; ++t)
-      if (tid + t * kBlock < nq * PV) pst[t] = *byte_at((const VT*)Vin, (voff[t] + (uint32_t)(e0 * NP * ES)) * (uint32_t)sizeof(T));
+      if (tid + t * kBlock < nq * PV) pst[t] = *(const VT*)tile_at(Vin, e0, NP * ES, voff[t]);
 #pragma unroll
     for (int t = 0; t < NGEO; ++t) {
       const int idx = tid + t * kBlock;
-      if (idx < TE * 9) pgeo[t] = *byte_at(Jinv, (uint32_t)(e0 * 9 + idx) * (uint32_t)sizeof(T));
+      if (idx < TE * 9) pgeo[t] = *tile_at(Jinv, e0, 9, (uint32_t)idx);
     }
 #pragma unroll
     for (int t = 0; t < NFC; ++t) {
       const int idx = tid + t * kBlock;
       if (idx < TE * NF)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) pfg[t][c] = *byte_at(Fg, (uint32_t)((e0 * NF + idx) * 4 + c) * (uint32_t)sizeof(T));
+        for (int c = 0; c < 4; ++c) pfg[t][c] = *tile_at(Fg, e0, NF * 4, (uint32_t)(idx * 4 + c));
     }
 #pragma unroll
     for (int t = 0; t < NI; ++t) {
       const int idx = tid + t * kBlock;
-      const int ej = e0 * NF + (idx / NFF) * NF + (idx % NFF) / NFP;
-      inc[t] = idx < TE * NFF ? make_int2(*byte_at(a.nbr, (uint32_t)ej * 4u), *byte_at(a.code, (uint32_t)ej * 4u))
+      const int ej = (idx / NFF) * NF + (idx % NFF) / NFP;
+      inc[t] = idx < TE * NFF ? make_int2(*tile_at(a.nbr, e0, NF, (uint32_t)ej), *tile_at(a.code, e0, NF, (uint32_t)ej))
                               : make_int2(-1, 0);
     }
   };
@@ -454,8 +495,15 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
     for (int t = 0; t < NI; ++t) {
       const int k = ((tid + t * kBlock) % NFF) % NFP;
       if (inc[t].x >= 0) {
-        const int pi = inc[t].x * NP + sVmap[(inc[t].y & 0xffff) * NFP + k];
-        node_fields<ES>(byte_at(Vin, (uint32_t)pi * (uint32_t)(ES * sizeof(T))), iqp[t]);
+        const uint32_t node = (uint32_t)sVmap[(inc[t].y & 0xffff) * NFP + k];
+        if constexpr (BIG) {
+          // the neighbour element's base in 64 bits, the node within it as a 32-bit offset
+          const T* nb = Vin + (uint64_t)(uint32_t)inc[t].x * (uint32_t)(NP * ES);
+          node_fields<ES>(byte_at(nb, node * (uint32_t)(ES * sizeof(T))), iqp[t]);
+        } else {
+          const uint32_t pi = (uint32_t)inc[t].x * (uint32_t)NP + node;
+          node_fields<ES>(byte_at(Vin, pi * (uint32_t)(ES * sizeof(T))), iqp[t]);
+        }
       } else {
 #pragma unroll
         for (int c = 0; c < 4; ++c) iqp[t][c] = (T)0;
@@ -472,16 +520,16 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   auto valid = [&](int it) { return it < tpx && xcd * tpx + it < ntiles; };
   __syncthreads();  // sVmap ready for the first gathers
   if (valid(lb)) {
-    prefetch((xcd * tpx + lb) * TE);
+    prefetch((int64_t)(xcd * tpx + lb) * TE);
     gather();
   }
   for (int it = lb; valid(it); it += bpx) {
-    const int e0 = (xcd * tpx + it) * TE;
+    const int64_t e0 = (int64_t)(xcd * tpx + it) * TE;
     __syncthreads();  // the previous tile is done with LDS
     // phase 1: commit the prefetched tile to LDS; epilogue operands and the next tile in flight
 #pragma unroll
     for (int t = 0; t < NE; ++t)
-      if (tid + t * kBlock < nq * PV) *(VT*)(&sU[0][0] + loff[t]) = pst[t];
+      if (tid + t * kBlock < nq * PV) *(VT*)(sU + loff[t]) = pst[t];
 #pragma unroll
     for (int t = 0; t < NGEO; ++t) {
       const int idx = tid + t * kBlock;
@@ -507,10 +555,9 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
 #pragma unroll
       for (int t = 0; t < NE; ++t)
         if (tid + t * kBlock < nq * PV) {
-          const uint32_t id = voff[t] + (uint32_t)(e0 * NP * ES), ib = id * (uint32_t)sizeof(T);
-          if (stage != 3 || CS) pu[t] = __builtin_nontemporal_load(byte_at((const VT*)U, ib));
-          if (CS && stage == 3) plo[CS ? t : 0] = *byte_at((const uint2*)Ulo, id * 2u);
-          pa[t] = __builtin_nontemporal_load(byte_at((const VT*)Acc, ib));
+          if (stage != 3 || CS) pu[t] = __builtin_nontemporal_load((const VT*)tile_at(U, e0, NP * ES, voff[t]));
+          if (CS && stage == 3) plo[CS ? t : 0] = *(const uint2*)tile_at(Ulo, e0, NP * ES, voff[t]);
+          pa[t] = __builtin_nontemporal_load((const VT*)tile_at(Acc, e0, NP * ES, voff[t]));
         }
     }
     };
@@ -521,7 +568,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
 #pragma unroll
     for (int t = 0; t < NI; ++t) cnc[t] = inc[t];
     const bool more = valid(it + bpx);
-    if (more) prefetch((xcd * tpx + it + bpx) * TE);
+    if (more) prefetch((int64_t)(xcd * tpx + it + bpx) * TE);
     __syncthreads();
     // phase 2: nodal normal numerical fluxes scaled by sJ/|detJ| -> sB[(e*4 + q)*RB + jk]. All three face
     // kinds share one branch-free form (no divergence between interior and boundary face nodes):
@@ -540,9 +587,8 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
       const int2 nc = cnc[t];
       const T n[3] = {sN[ej][0], sN[ej][1], sN[ej][2]};
       const T fs = sN[ej][3];
-      const int on = e * NP + sFmask[jk];
       T qm[4];
-      node_fields<ES>(&sU[0][0] + on * ES, qm);
+      node_fields<ES>(sU + e * LES + sFmask[jk] * ES, qm);
       const T vn = v0[0] * n[0] + v0[1] * n[1] + v0[2] * n[2];
       const T unm = qm[1] * n[0] + qm[2] * n[1] + qm[3] * n[2];
       const bool inter = nc.x >= 0, refl = nc.x == -2;
@@ -578,10 +624,10 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
         }
       }
     }
-    V acc[MBF ? MBF : 1];
+    V acc[MBW ? MBW * NCH : 1];
     typename Tail<T>::A acct[NT4 ? NT4 : 1];
 #pragma unroll
-    for (int mb = 0; mb < MBF; ++mb) acc[mb] = V{0, 0, 0, 0};
+    for (int mb = 0; mb < MBW * NCH; ++mb) acc[mb] = V{0, 0, 0, 0};
 #pragma unroll
     for (int t = 0; t < NT4; ++t) acct[t] = typename Tail<T>::A{};
     bf16x8 blo;
@@ -593,7 +639,10 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
     if (!kALds) asm volatile("" : "+v"(pAt), "+v"(pTt));
     auto step = [&](const int s, const T b) {
 #pragma unroll
-      for (int mb = 0; mb < MBF; ++mb) acc[mb] = Mfma<T>::mma(pAt[(mb * KS + s) * 64], b, acc[mb]);
+      for (int mb = 0; mb < MBW; ++mb) {
+        const int ai = NCH > 1 ? mb * NCH + (s & 1) : mb;
+        acc[ai] = Mfma<T>::mma(pAt[(mb * KS + s) * 64], b, acc[ai]);
+      }
 #pragma unroll
       for (int t = 0; t < NT4; ++t) acct[t] = Tail<T>::mma(pTt[(t * KS + s) * 16], b, acct[t]);
       if constexpr (LO) {
@@ -614,7 +663,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
       T p = 0, vx = 0, vy = 0, vz = 0;
       if (n < NP) {
         T f[4];
-        node_fields<ES>(&sU[0][0] + (el * NP + n) * ES, f);
+        node_fields<ES>(sU + el * LES + n * ES, f);
         p = f[0];
         vx = f[1];
         vy = f[2];
@@ -633,12 +682,15 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
     }
     __syncthreads();  // face fluxes consumed; sB now receives the results in the LDS tile's layout
 #pragma unroll
-    for (int mb = 0; mb < MBF; ++mb)
+    for (int mb = 0; mb < MBW; ++mb) {
+      V r = acc[mb * NCH];
+      if constexpr (NCH > 1) r += acc[mb * NCH + 1];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int i = 16 * mb + d_row<T>(lane, j);
-        if (i < NP && q < nq) sR[(el * NP + i) * ES + q] = acc[mb][j];
+        const int i = 16 * (rg * MBW + mb) + d_row<T>(lane, j);
+        if (i < NP && q < nq) sR[el * LES + i * ES + q] = r[j];
       }
+    }
 #pragma unroll
     for (int t = 0; t < NT4; ++t) {
       if constexpr (sizeof(T) == 4) {
@@ -649,11 +701,11 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
           v += __shfl_xor(v, 16);
           v += __shfl_xor(v, 32);
           const int i = 16 * MBF + 4 * t + j;
-          if (g == 0 && i < NP && q < nq) sR[(el * NP + i) * ES + q] = v;
+          if (g == 0 && i < NP && q < nq) sR[el * LES + i * ES + q] = v;
         }
       } else {
         const int i = 16 * MBF + 4 * t + g;  // lane group g holds row g of the piece
-        if (i < NP && q < nq) sR[(el * NP + i) * ES + q] = (T)acct[t];
+        if (i < NP && q < nq) sR[el * LES + i * ES + q] = (T)acct[t];
       }
     }
     __syncthreads();
@@ -664,14 +716,13 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
     for (int t = 0; t < NE; ++t) {
       if (tid + t * kBlock >= nq * PV) break;
       const VT kq = dt * *(const VT*)(sR + loff[t]);
-      const uint32_t id = voff[t] + (uint32_t)(e0 * NP * ES), ib = id * (uint32_t)sizeof(T);
-      VT* const pAcc = byte_at((VT*)Acc, ib);
-      VT* const pOut = byte_at((VT*)Vout, ib);
-      VT* const pU = byte_at((VT*)U, ib);
-      uint2* const pLo = byte_at((uint2*)Ulo, id * 2u);
+      VT* const pAcc = (VT*)tile_at(Acc, e0, NP * ES, voff[t]);
+      VT* const pOut = (VT*)tile_at(Vout, e0, NP * ES, voff[t]);
+      VT* const pU = (VT*)tile_at(U, e0, NP * ES, voff[t]);
+      uint2* const pLo = (uint2*)tile_at(Ulo, e0, NP * ES, voff[t]);
       switch (stage) {
         case 0: {
-          const VT u = *(const VT*)(&sU[0][0] + loff[t]);
+          const VT u = *(const VT*)(sU + loff[t]);
           __builtin_nontemporal_store(CS ? kq * s6 : u + kq * s6, pAcc);
           *pOut = u + kq * s2;
         } break;
@@ -702,7 +753,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
           break;
         case 4: *pOut = kq; break;
         default:
-          *pOut = *(const VT*)(&sU[0][0] + loff[t]) + kq;
+          *pOut = *(const VT*)(sU + loff[t]) + kq;
           if (CS) *pLo = make_uint2(0, 0);  // Vout becomes U: an Euler step drops the low word
           break;
       }
@@ -713,21 +764,41 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
 // idx holds node offsets e*Np + node; the pressure sits at (e*Np + node)*es, es being the
 // interleaved field count (1 for field planes)
 template <typename T>
-__global__ void k_source(T* __restrict__ U, uint16_t* __restrict__ Ulo, const int* __restrict__ idx, int64_t n,
+__global__ void k_source(T* __restrict__ U, uint16_t* __restrict__ Ulo, const int64_t* __restrict__ idx, int64_t n,
                          T value, int es) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) {
-    const int64_t j = (int64_t)idx[i] * es;
+    const int64_t j = idx[i] * es;
     U[j] = value;
     if (Ulo) Ulo[j] = 0;
   }
+}
+
+// The same overwrite at the step whose start time is tdev[*tstep], when it lies before the
+// source's end: amp*sin(2 pi freq t + phase) (src/solver.cpp:248-256), evaluated in FP64
+template <typename T>
+__global__ void k_source_timed(T* __restrict__ U, uint16_t* __restrict__ Ulo, const int64_t* __restrict__ idx,
+                               int64_t n, double amp, double freq, double phase, double duration,
+                               const double* __restrict__ tdev, const int* __restrict__ tstep, int es) {
+  const double t = tdev[*tstep];
+  if (!(t < duration)) return;
+  const T value = (T)(amp * sin(2 * M_PI * freq * t + phase));
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = idx[i] * es;
+    U[j] = value;
+    if (Ulo) Ulo[j] = 0;
+  }
+}
+
+__global__ void k_tick(int* __restrict__ tstep) {
+  if (threadIdx.x == 0) *tstep += 1;
 }
 
 // One workgroup: every thread interpolates receivers r = tid, tid+256, ... from the pressure field
 // (value (e*Np + node)*es of the state buffer) and writes row *counter of the record; the row index is read before and advanced after the
 // writes, so a captured graph records consecutive rows on every replay. Rows beyond cap are dropped.
 template <typename T>
-__global__ void k_record(const T* __restrict__ U, const uint16_t* __restrict__ Ulo, const int* __restrict__ base,
+__global__ void k_record(const T* __restrict__ U, const uint16_t* __restrict__ Ulo, const int64_t* __restrict__ base,
                          const T* __restrict__ w, int n, int Np, T* __restrict__ out, int* __restrict__ counter,
                          int cap, int es) {
   __shared__ int row;
@@ -736,7 +807,7 @@ __global__ void k_record(const T* __restrict__ U, const uint16_t* __restrict__ U
   const int r0 = row;
   if (r0 < cap) {
     for (int r = threadIdx.x; r < n; r += blockDim.x) {
-      const int b = base[r];
+      const int64_t b = base[r];
       T acc = 0;
       for (int k = 0; k < Np; ++k) {
         const int64_t j = (int64_t)(b + k) * es;  // pressure of node k
@@ -830,12 +901,12 @@ void launch_lds(StageArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((k_stage_lds<T, D, P>), dim3(8 * bpx), dim3(kBlock), 0, s, a);
 }
 
-template <typename T, int D, int P, bool LO, bool CS, int ES>
-void launch_mfma_es(StageArgs& a, hipStream_t s, int grid_pct) {
+template <typename T, int D, int P, bool LO, bool CS, int ES, bool BIG>
+void launch_mfma_kernel(StageArgs& a, hipStream_t s, int grid_pct) {
   constexpr int TE = MfmaLaunch<T, D, P>::TE, KB = MfmaLaunch<T, D, P>::KB;
   static int occ = 0;
   if (!occ) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stage_mfma<T, D, P, LO, CS, ES>, KB, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stage_mfma<T, D, P, LO, CS, ES, BIG>, KB, 0) != hipSuccess ||
         occ < 1)
       occ = 1;
   }
@@ -844,7 +915,19 @@ void launch_mfma_es(StageArgs& a, hipStream_t s, int grid_pct) {
   const int64_t per_xcd_cap = (int64_t)(num_cus() / 8) * occ * std::max(grid_pct, 1) / 100;
   int bpx = (int)std::min<int64_t>(a.tiles_per_xcd, std::max<int64_t>(per_xcd_cap, 1));
   if (bpx < 1) bpx = 1;
-  hipLaunchKernelGGL((k_stage_mfma<T, D, P, LO, CS, ES>), dim3(8 * bpx), dim3(KB), 0, s, a);
+  hipLaunchKernelGGL((k_stage_mfma<T, D, P, LO, CS, ES, BIG>), dim3(8 * bpx), dim3(KB), 0, s, a);
+}
+
+// 32-bit offsets from the buffers' own bases while every array the kernel addresses (state,
+// J^-1, face geometry; the face tables are smaller) stays below 4 GiB, else per-tile 64-bit bases
+template <typename T, int D, int P, bool LO, bool CS, int ES>
+void launch_mfma_es(StageArgs& a, hipStream_t s, int grid_pct) {
+  const uint64_t lim = (uint64_t)1 << 32, kp = (uint64_t)a.Kpad;
+  const bool big = a.addr64 || kp * Sz<D, P>::NP * ES * sizeof(T) >= lim || kp * 9 * sizeof(T) >= lim ||
+                   kp * Sz<D, P>::NF * 4 * sizeof(T) >= lim;
+  if (!big) return launch_mfma_kernel<T, D, P, LO, CS, ES, false>(a, s, grid_pct);
+  if constexpr (LO) throw Unsupported("the FP32 operator correction is limited to contexts below 4 GiB per buffer");
+  else launch_mfma_kernel<T, D, P, LO, CS, ES, true>(a, s, grid_pct);
 }
 
 // MFMA contexts keep the state node-interleaved with their live fields: ES = 4 in 3D, and in
@@ -896,6 +979,25 @@ void launch_source(Ctx& c, const Ctx::Src& src, double value, hipStream_t st) {
   else
     hipLaunchKernelGGL(k_source<float>, dim3(nb), dim3(256), 0, st, (float*)c.U, (uint16_t*)(c.comp ? c.Ulo : nullptr),
                        src.idx, src.n, (float)value, c.es);
+  DGK_HIP(hipGetLastError());
+}
+
+void launch_source_timed(Ctx& c, const Ctx::Src& src, hipStream_t st) {
+  if (!src.n) return;
+  const unsigned nb = (unsigned)std::min<int64_t>((src.n + 255) / 256, 1024);
+  if (c.precision == DGK_FP64)
+    hipLaunchKernelGGL(k_source_timed<double>, dim3(nb), dim3(256), 0, st, (double*)c.U, (uint16_t*)nullptr, src.idx,
+                       src.n, src.amp, src.freq, src.phase, src.duration, (const double*)c.tdev,
+                       (const int*)c.tstep, c.es);
+  else
+    hipLaunchKernelGGL(k_source_timed<float>, dim3(nb), dim3(256), 0, st, (float*)c.U,
+                       (uint16_t*)(c.comp ? c.Ulo : nullptr), src.idx, src.n, src.amp, src.freq, src.phase,
+                       src.duration, (const double*)c.tdev, (const int*)c.tstep, c.es);
+  DGK_HIP(hipGetLastError());
+}
+
+void launch_tick(Ctx& c, hipStream_t st) {
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, st, c.tstep);
   DGK_HIP(hipGetLastError());
 }
 

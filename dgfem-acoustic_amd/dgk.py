@@ -177,7 +177,8 @@ class _Phys(ctypes.Structure):
 
 class _Opts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("flux_sign", ctypes.c_int), ("device", ctypes.c_int),
-                ("variant", ctypes.c_int), ("fp32_flags", ctypes.c_int), ("reserved", ctypes.c_int * 5)]
+                ("variant", ctypes.c_int), ("fp32_flags", ctypes.c_int), ("addr64", ctypes.c_int),
+                ("reserved", ctypes.c_int * 4)]
 
 
 class _Info(ctypes.Structure):
@@ -258,7 +259,7 @@ class Solver:
     """One libdgk context: the reference's Mesh + solver state on the GPU."""
 
     def __init__(self, mesh, c0, rho0, v0=(0.0, 0.0, 0.0), precision=FP64, flux_sign=SIGN_REFERENCE, device_id=0,
-                 variant=0, fp32_flags=0):
+                 variant=0, fp32_flags=0, addr64=False):
         L = device()
         self.mesh = mesh
         self.K, self.Np = mesh.K, mesh.Np
@@ -268,7 +269,7 @@ class Solver:
         m = _Mesh(mesh.dim, mesh.order, mesh.K, _p(self._xyz), _p(self._tags, ctypes.c_int64),
                   _p(self._bc, ctypes.c_uint8))
         ph = _Phys(float(c0), float(rho0), (ctypes.c_double * 3)(*[float(x) for x in v0]))
-        op = _Opts(int(precision), int(flux_sign), int(device_id), int(variant), int(fp32_flags))
+        op = _Opts(int(precision), int(flux_sign), int(device_id), int(variant), int(fp32_flags), int(bool(addr64)))
         h = ctypes.c_void_p()
         st = L.dgk_create(ctypes.byref(m), ctypes.byref(ph), ctypes.byref(op), ctypes.byref(h))
         if st:

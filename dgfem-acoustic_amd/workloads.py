@@ -77,11 +77,9 @@ def room_3d(h=0.25, order=4, seed=4):
     return m, phys, dt, u0
 
 
-def ensemble_member(i, seed0=7, n=256, kmin=5000, kmax=80000, orders=(2, 3, 4, 5)):
-    """Member i of the uneven ensemble: the first half 2D rectangles (2 triangles per cell), the
-    rest 3D boxes (Kuhn split), sides U[1,3], K ~ U[kmin,kmax], order ~ U{orders}, each wall
-    reflecting or absorbing with probability 1/2, Gaussian pulse with sigma ~ U[0.02,0.08] times
-    the shortest side, centred at least 3 sigma from every wall. Seeds 7+i (MT19937)."""
+def ensemble_draw(i, seed0=7, n=256, kmin=5000, kmax=80000, orders=(2, 3, 4, 5)):
+    """The random draws of ensemble member i, without building its mesh: (dim, K_target, order,
+    sides L, wall groups, sigma, centre)."""
     rng = np.random.Generator(np.random.MT19937(seed0 + i))
     dim = 2 if i < n // 2 else 3
     K_target = int(rng.integers(kmin, kmax + 1))
@@ -90,6 +88,18 @@ def ensemble_member(i, seed0=7, n=256, kmin=5000, kmax=80000, orders=(2, 3, 4, 5
     walls = [1 if r < 0.5 else 2 for r in rng.random(2 * dim)]
     sigma = (0.02 + 0.06 * rng.random()) * float(L.min())
     center = [3 * sigma + (L[k] - 6 * sigma) * rng.random() for k in range(dim)] + [0.0] * (3 - dim)
+    return dim, K_target, p, L, walls, sigma, center
+
+
+def ensemble_member(i, seed0=7, n=256, kmin=5000, kmax=80000, orders=(2, 3, 4, 5), k_target=None):
+    """Member i of the uneven ensemble: the first half 2D rectangles (2 triangles per cell), the
+    rest 3D boxes (Kuhn split), sides U[1,3], K ~ U[kmin,kmax], order ~ U{orders}, each wall
+    reflecting or absorbing with probability 1/2, Gaussian pulse with sigma ~ U[0.02,0.08] times
+    the shortest side, centred at least 3 sigma from every wall. Seeds 7+i (MT19937). k_target
+    replaces the drawn element count (a smaller copy of the member: same shape, walls and pulse)."""
+    dim, K_target, p, L, walls, sigma, center = ensemble_draw(i, seed0, n, kmin, kmax, orders)
+    if k_target is not None:
+        K_target = k_target
     names = "1:Wall;2:Abs"
     text = conf("Wall=Reflecting\nAbs=Absorbing\n")
     if dim == 2:
@@ -108,13 +118,13 @@ def ensemble_member(i, seed0=7, n=256, kmin=5000, kmax=80000, orders=(2, 3, 4, 5
     return m, phys, dt, center, sigma
 
 
-def ensemble(n=256, seed0=7, kmin=5000, kmax=80000, orders=(2, 3, 4, 5), threads=16, members=None):
+def ensemble(n=256, seed0=7, kmin=5000, kmax=80000, orders=(2, 3, 4, 5), threads=16, members=None, k_target=None):
     """The uneven ensemble (see ensemble_member), generated on a thread pool (the generators
     release the GIL). Returns lists (meshes, phys, dts, pulses); ensemble_state(mesh, pulse)
     builds a member's initial state."""
     idx = list(range(n)) if members is None else list(members)
     with ThreadPoolExecutor(max_workers=threads) as ex:
-        out = list(ex.map(lambda i: ensemble_member(i, seed0, n, kmin, kmax, orders), idx))
+        out = list(ex.map(lambda i: ensemble_member(i, seed0, n, kmin, kmax, orders, k_target), idx))
     return [o[0] for o in out], [o[1] for o in out], [o[2] for o in out], [(o[3], o[4]) for o in out]
 
 

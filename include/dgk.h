@@ -60,7 +60,9 @@ typedef struct {
   int device;      /* HIP device ordinal */
   int variant;     /* stage-kernel variant, 0 = autotuned/default (see dgk_variant_name) */
   int fp32_flags;  /* DGK_FP32 only: DGK_FP32_* bits, 0 = the default FP32 form */
-  int reserved[5];
+  int addr64;      /* 1: the stage kernel addresses every array through 64-bit per-tile bases even
+                      when all fit 32-bit offsets (0: only where an array reaches 4 GiB) */
+  int reserved[4];
 } dgk_opts;
 
 /* fp32_flags bits. By default the FP32 form keeps the solution u as an FP32 value plus a bf16
@@ -163,7 +165,8 @@ dgk_status dgk_batch_set_receivers(dgk_batch* b, int sim, int n, const double* x
 dgk_status dgk_batch_get_receivers(dgk_batch* b, int sim, double* out, int capacity_rows, int* rows);
 /* launch shape of one group's stage kernel (see dgk_set_launch) */
 dgk_status dgk_batch_set_launch(dgk_batch* b, int group, int grid_pct, int tile_order);
-/* capture (and keep) the graph that dgk_batch_rk4(b, nsteps, schedule) will replay, without running it */
+/* capture (and keep) the graphs that dgk_batch_rk4(b, nsteps, schedule) will replay (up to 100 steps
+ * each, the remainder in a graph of its own length), without running them */
 dgk_status dgk_batch_prepare(dgk_batch* b, int nsteps, int schedule);
 /* nsteps RK4 steps group by group without a graph, with a HIP event pair around every stage
  * launch; group_stage_ms[g] receives the summed stage-kernel time of group g */

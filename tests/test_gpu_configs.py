@@ -1,0 +1,140 @@
+"""GPU parity of the benchmark workloads as their generators build them.
+
+* The uneven ensemble (256 members: 2D rectangles and 3D boxes, orders 2-5, mixed walls, own
+  pulse and time step each): for every (dimension, order) the smallest of the 256 members, run
+  together as one batch, and a 16-member ensemble drawn by the same generator at 500-3000
+  elements, each member against its own oracle run (one reference `rungeKutta` run,
+  src/solver.cpp:171-292).
+* The room with its block obstacle, Walls/Ceiling split and 8 seeded receivers, at h = 1 m
+  (14,376 tets, order 4), FP64 and FP32 against the oracle for 100 steps.
+* The 3D P3 cube at full size (196,608 tets, mean flow, absorbing walls) for 50 steps, FP64
+  and FP32, against the oracle.
+
+FP64 bar 1e-12 relative L2 over the whole state (the drop-in contract); FP32 1e-5 per field.
+Per field the FP64 velocities of order-5 members reach about 1e-12 after 30 steps: that is the
+floating-point floor of the operator itself, which the oracle's literal (quadrature-point) and
+factorized evaluations of the same reference algorithm also show between each other (1.6e-12 on
+ensemble member 43, 2D order 5, 30 steps).
+"""
+import numpy as np
+import pytest
+
+import dgk
+import oracle_binding as ob
+import workloads as wl
+from interp import sample
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 30
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def smallest_member_per_shape(n=256):
+    """Index of the smallest member (by drawn element count) of every (dim, order) of the ensemble."""
+    best = {}
+    for i in range(n):
+        dim, K, p = wl.ensemble_draw(i, n=n)[:3]
+        if (dim, p) not in best or K < best[(dim, p)][0]:
+            best[(dim, p)] = (K, i)
+    return sorted(i for _, i in best.values())
+
+
+def run_batch_against_oracle(meshes, phys, dts, pulses, label):
+    refs = []
+    for m, ph, dt, pu in zip(meshes, phys, dts, pulses):
+        o = ob.Oracle(m, ph["c0"], ph["rho0"], v0=ph["v0"], sign_mode=ob.SIGN_STABLE, factorized_only=True)
+        refs.append(o.step(wl.ensemble_state(m, pu), STEPS, dt))
+    for prec, bar in ((dgk.FP64, 1e-12), (dgk.FP32, 1e-5)):
+        print(f"{label}: {len(meshes)} members, precision {prec}, {STEPS} steps", flush=True)
+        b = dgk.Batch(meshes, phys, dts, precision=prec, flux_sign=dgk.SIGN_STABLE)
+        for i, (m, pu) in enumerate(zip(meshes, pulses)):
+            b.set_state(i, wl.ensemble_state(m, pu))
+        b.rk4(STEPS)
+        worst = 0.0
+        for i, (m, ref) in enumerate(zip(meshes, refs)):
+            g = b.get_state(i)
+            errs = [rel(g[q], ref[q]) for q in range(m.dim + 1)]
+            err = rel(g, ref) if prec == dgk.FP64 else max(errs)
+            worst = max(worst, err)
+            print(f"  member {i}: dim {m.dim} order {m.order} K {m.K} per field {errs}", flush=True)
+            assert err <= bar, (label, prec, i, m.dim, m.order, m.K, errs)
+            assert not g[m.dim + 1:].any()
+        print(f"{label}: precision {prec} worst member rel-L2 {worst:.2e}", flush=True)
+        b.close()
+
+
+def test_ensemble256_smallest_member_of_every_shape():
+    idx = smallest_member_per_shape()
+    assert len(idx) == 8  # 2D and 3D at orders 2-5
+    meshes, phys, dts, pulses = wl.ensemble(n=256, members=idx, threads=8)
+    assert sorted((m.dim, m.order) for m in meshes) == [(d, p) for d in (2, 3) for p in (2, 3, 4, 5)]
+    assert all(5000 * 0.8 <= m.K <= 80000 * 1.2 for m in meshes)
+    run_batch_against_oracle(meshes, phys, dts, pulses, "ensemble256 members " + ",".join(map(str, idx)))
+
+
+def test_ensemble_generator_16_members():
+    meshes, phys, dts, pulses = wl.ensemble(n=16, kmin=500, kmax=3000, threads=8)
+    assert sum(m.dim == 2 for m in meshes) == 8
+    run_batch_against_oracle(meshes, phys, dts, pulses, "ensemble16")
+
+
+def room_receivers(m, center, seed=6):
+    """8 receivers: 4 within 0.6 m of the pulse centre, 4 anywhere above the block height."""
+    rng = np.random.default_rng(seed)
+    near = np.asarray(center) + 1.2 * (rng.random((4, 3)) - 0.5)
+    far = np.array([1.0, 1.0, 1.5]) + np.array([18.0, 13.0, 5.5]) * rng.random((4, 3))
+    return np.vstack([near, far])
+
+
+def test_room_block_walls_ceiling_100_steps_with_receivers():
+    """Config-3 geometry (block obstacle on the floor, reflecting walls, absorbing ceiling) at
+    h = 1 m and order 4, against the oracle after 100 steps, fields and receivers."""
+    m, ph, dt, u0 = wl.room_3d(h=1.0)
+    assert m.Np == 35 and m.K == 20 * 15 * 8 * 6 - 2 * 2 * 1 * 6
+    assert sorted(g[1] for g in m.bc_groups) == [0, 1]
+    center = m.node_xyz[np.argmax(u0[0])]
+    pts = room_receivers(m, center)
+    o = ob.Oracle(m, ph["c0"], ph["rho0"], sign_mode=ob.SIGN_STABLE, factorized_only=True)
+    nsteps = 100
+    u, rows = u0, []
+    for _ in range(nsteps):
+        u = o.step(u, 1, dt)
+        rows.append(sample(m, u, pts))
+    ref_rec = np.array(rows)
+    assert np.abs(ref_rec[:, :4]).max() > 1e-6  # the near receivers see the pulse
+    for prec, bar in ((dgk.FP64, 1e-12), (dgk.FP32, 1e-5)):
+        print(f"room h=1 order 4: precision {prec}, {nsteps} steps, 8 receivers", flush=True)
+        s = dgk.Solver(m, ph["c0"], ph["rho0"], precision=prec, flux_sign=dgk.SIGN_STABLE)
+        s.set_state(u0)
+        s.set_receivers(pts, nsteps)
+        s.rk4(nsteps, dt)
+        g = s.get_state()
+        errs = [rel(g[q], u[q]) for q in range(4)]
+        rec = s.get_receivers()
+        print(f"  fields {errs}, receivers {rel(rec, ref_rec):.2e}", flush=True)
+        assert (rel(g, u) if prec == dgk.FP64 else max(errs)) <= bar
+        assert rel(rec, ref_rec) <= (1e-12 if prec == dgk.FP64 else 1e-4)
+        s.close()
+
+
+def test_cube_3d_full_size_50_steps():
+    """The 3D P3 workload at full size (196,608 tets, v0 = (50,0,0), absorbing walls): 50 steps
+    against the oracle, FP64 and FP32."""
+    m, ph, dt, u0 = wl.cube_3d()
+    assert m.K == 196608
+    o = ob.Oracle(m, ph["c0"], ph["rho0"], v0=ph["v0"], sign_mode=ob.SIGN_STABLE, factorized_only=True)
+    ref = o.step(u0, 50, dt)
+    for prec, bar in ((dgk.FP64, 1e-12), (dgk.FP32, 1e-5)):
+        print(f"cube 196608 tets: precision {prec}, 50 steps", flush=True)
+        s = dgk.Solver(m, ph["c0"], ph["rho0"], v0=ph["v0"], precision=prec, flux_sign=dgk.SIGN_STABLE)
+        s.set_state(u0)
+        s.rk4(50, dt)
+        g = s.get_state()
+        errs = [rel(g[q], ref[q]) for q in range(4)]
+        print(f"  {errs}", flush=True)
+        assert (rel(g, ref) if prec == dgk.FP64 else max(errs)) <= bar
+        s.close()

@@ -130,6 +130,30 @@ def test_euler_and_sources_fp64(dim):
     assert rel(s.get_state(), o.step(u0, 30, dt, t_steps=ts[:30], sources=src, euler=True)) <= 1e-12
 
 
+@pytest.mark.parametrize("prec", [dgk.FP64, dgk.FP32])
+def test_sources_replayed_from_graphs_across_calls(prec):
+    """Source steps run from captured graphs (the source kernels read the step's start time on the
+    device): calls of 7, 25, 1 and 40 steps in a row, a source that ends inside the second call and
+    one that lasts, against one oracle run over all 73 steps."""
+    print(f"sources through graphs, precision {prec}", flush=True)
+    m = mixed_mesh(2, 3, 8)
+    o = ob.Oracle(m, 340.0, 1.225, v0=(5.0, 0.0, 0.0))
+    s = dgk.Solver(m, 340.0, 1.225, v0=(5.0, 0.0, 0.0), precision=prec)
+    dt = 1e-6
+    ts = np.cumsum(np.full(73, dt)) - dt
+    src = np.array([[0, 0.5, 0.4, 0.0, 0.2, 1.0, 800.0, 0.3, ts[20] + 0.5 * dt],
+                    [0, 0.3, 0.3, 0.0, 0.15, 2.0, 500.0, 0.0, 1.0]])
+    u0 = np.zeros((4, m.K * m.Np))
+    s.set_sources(src)
+    s.set_state(u0)
+    i = 0
+    for n in (7, 25, 1, 40):
+        s.rk4(n, dt, ts[i:i + n])
+        i += n
+    ref = o.step(u0, 73, dt, t_steps=ts, sources=src)
+    assert rel(s.get_state(), ref) <= (1e-12 if prec == dgk.FP64 else 1e-5)
+
+
 def test_cube_3d_full_size_fp64():
     """The 3D P3 workload at full size (196,608 tets): 3 steps against the oracle, then finite."""
     m, ph, dt, u0 = wl.cube_3d()
