@@ -101,6 +101,28 @@ def measured_traffic(kernel, workload):
     return None
 
 
+def measured_mfma_busy(kernel, workload):
+    """Fraction of the matrix cores' cycles busy during `kernel` on `workload`, from the rocprofv3
+    counter summaries under profiles/ (table rows "| kernel | workload | SQ_VALU_MFMA_BUSY_CYCLES |
+    GRBM_GUI_ACTIVE | CUs | busy |", busy = MFMA busy cycles / (GUI-active cycles x 4 SIMDs x CUs)),
+    or None."""
+    d = os.path.join(ROOT, "profiles")
+    key = kernel.split(">")[0].replace(" ", "")
+    if not os.path.isdir(d):
+        return None
+    for f in sorted(os.listdir(d)):
+        if not f.endswith(".md"):
+            continue
+        for line in open(os.path.join(d, f)):
+            cols = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cols) == 6 and cols[0].replace(" ", "").startswith(key + ",") and cols[1] == workload:
+                try:
+                    return {"frac": float(cols[5]), "source": "profiles/" + f}
+                except ValueError:
+                    continue
+    return None
+
+
 def roofline(bytes_per_launch, flops_per_launch, word, stage_s, kernel, workload):
     """The bound is the roof the kernel's algorithmic intensity (flops per HBM byte) meets first:
     HBM below the ridge MFMA_PEAK/HBM_PEAK, the matrix cores above it."""
@@ -123,6 +145,10 @@ def roofline(bytes_per_launch, flops_per_launch, word, stage_s, kernel, workload
     if tr:
         out["traffic_source"] = tr["source"]
         out["traffic_GBps"] = round(tr["bytes"] / stage_s / 1e9, 1)
+    mb = measured_mfma_busy(kernel, workload)
+    out["mfma_busy"] = mb["frac"] if mb else None
+    if mb:
+        out["mfma_busy_source"] = mb["source"]
     return out
 
 

@@ -547,6 +547,19 @@ void install_receivers(Ctx& c, const std::vector<int64_t>& elem, const std::vect
   DGK_HIP(hipMemset(c.rec.counter, 0, sizeof(int)));
 }
 
+// Put rows x rec.n host values back into a freshly installed record and continue after them.
+void restore_receivers(Ctx& c, const std::vector<double>& h, int rows) {
+  const size_t m = (size_t)rows * c.rec.n;
+  if (rows > c.rec.cap || h.size() < m) throw InvalidArg("restored receiver rows exceed the record");
+  if (c.precision == DGK_FP64) {
+    DGK_HIP(hipMemcpy(c.rec.out, h.data(), m * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> f(h.begin(), h.begin() + m);
+    DGK_HIP(hipMemcpy(c.rec.out, f.data(), m * sizeof(float), hipMemcpyHostToDevice));
+  }
+  DGK_HIP(hipMemcpy(c.rec.counter, &rows, sizeof(int), hipMemcpyHostToDevice));
+}
+
 // Copy the recorded rows (all receivers) to host doubles; returns the row count.
 int fetch_receivers(Ctx& c, std::vector<double>& out) {
   int rows = 0;
@@ -973,6 +986,7 @@ struct dgk_batch {
     // receivers per member: element (member-local) and basis weights; columns in the group record
     std::vector<std::vector<int64_t>> rel;
     std::vector<std::vector<double>> rw;
+    std::vector<int> rstart;  // record row at which each member's series begins
     int rcap = 0;
   };
   std::vector<Group> groups;
@@ -1260,21 +1274,43 @@ dgk_status dgk_batch_set_receivers(dgk_batch* b, int sim, int n, const double* x
     auto w = b->where[sim];
     auto& G = b->groups[w.first];
     dgk::Ctx& c = *G.ctx;
-    G.rel.resize(G.sims.size());
-    G.rw.resize(G.sims.size());
+    const size_t nm = G.sims.size();
+    G.rel.resize(nm);
+    G.rw.resize(nm);
+    G.rstart.resize(nm, 0);
+    if (n > 0 && capacity < 1) throw dgk::InvalidArg("receiver capacity must be positive");
+    // rows recorded so far for the other members are kept; this member's series restarts
+    std::vector<double> old;
+    const int rows = dgk::fetch_receivers(c, old);
+    std::vector<size_t> col0(nm, 0);
+    for (size_t k = 1; k < nm; ++k) col0[k] = col0[k - 1] + G.rel[k - 1].size();
+    const size_t nold = c.rec.n;
     dgk::RefElem R = dgk::make_ref_elem(c.dim, c.p);
     std::vector<double> wt;
     std::vector<int64_t> el = dgk::locate_points(R, node_xyz, b->K[sim], n, xyz, wt);
     G.rel[w.second] = el;
     G.rw[w.second] = wt;
-    G.rcap = std::max(G.rcap, capacity);
+    G.rstart[w.second] = rows;
+    G.rcap = std::max(G.rcap, rows + capacity);  // room for `capacity` more steps
     std::vector<int64_t> all;
     std::vector<double> allw;
-    for (size_t k = 0; k < G.sims.size(); ++k) {
+    for (size_t k = 0; k < nm; ++k) {
       for (int64_t e : G.rel[k]) all.push_back(e + G.off[k]);
       allw.insert(allw.end(), G.rw[k].begin(), G.rw[k].end());
     }
+    std::vector<double> keep;
+    if (rows > 0 && !all.empty()) {
+      keep.assign((size_t)rows * all.size(), std::nan(""));
+      size_t c1 = 0;
+      for (size_t k = 0; k < nm; ++k) {
+        if ((int)k != w.second)
+          for (int i = 0; i < rows; ++i)
+            for (size_t j = 0; j < G.rel[k].size(); ++j) keep[(size_t)i * all.size() + c1 + j] = old[(size_t)i * nold + col0[k] + j];
+        c1 += G.rel[k].size();
+      }
+    }
     dgk::install_receivers(c, all, allw, G.rcap);
+    if (!keep.empty()) dgk::restore_receivers(c, keep, rows);
     b->drop_graphs();
   });
 }
@@ -1292,12 +1328,13 @@ dgk_status dgk_batch_get_receivers(dgk_batch* b, int sim, double* out, int capac
       for (int k = 0; k < w.second; ++k) col0 += G.rel[k].size();
       n = G.rel[w.second].size();
     }
-    *rows = n ? r : 0;
+    const int r0 = n ? std::min(G.rstart[w.second], r) : 0;  // the member's series begins at row r0
+    *rows = n ? r - r0 : 0;
     if (out && n) {
-      if (r > capacity_rows) throw dgk::InvalidArg("output buffer holds fewer rows than recorded");
+      if (r - r0 > capacity_rows) throw dgk::InvalidArg("output buffer holds fewer rows than recorded");
       const size_t ng = G.ctx->rec.n;
-      for (int i = 0; i < r; ++i)
-        for (size_t j = 0; j < n; ++j) out[(size_t)i * n + j] = h[(size_t)i * ng + col0 + j];
+      for (int i = r0; i < r; ++i)
+        for (size_t j = 0; j < n; ++j) out[(size_t)(i - r0) * n + j] = h[(size_t)i * ng + col0 + j];
     }
   });
 }

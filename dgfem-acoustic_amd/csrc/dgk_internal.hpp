@@ -191,6 +191,7 @@ std::vector<int64_t> locate_points(const RefElem& R, const double* node_xyz, int
                                    std::vector<double>& w);
 void install_receivers(Ctx& c, const std::vector<int64_t>& elem, const std::vector<double>& w, int cap);
 int fetch_receivers(Ctx& c, std::vector<double>& out);
+void restore_receivers(Ctx& c, const std::vector<double>& rows_host, int rows);
 
 
 }  // namespace dgk

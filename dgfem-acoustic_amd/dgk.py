@@ -422,7 +422,8 @@ class Batch:
         self._check(device().dgk_batch_sync(self.h))
 
     def set_receivers(self, i, xyz, capacity):
-        """Record member i's pressure at points xyz (n x 3) after every step (group capacity = max asked)."""
+        """Record member i's pressure at points xyz (n x 3) after every following step, for at least
+        `capacity` steps; rows already recorded for other members of its group are kept."""
         m = self.meshes[i]
         x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
         nx = np.ascontiguousarray(m.node_xyz, dtype=np.float64)

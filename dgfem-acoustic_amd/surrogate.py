@@ -165,6 +165,54 @@ def dg_impulse_responses(mesh, phys, dt, sources, receivers, nsteps, sigma, stri
     return out
 
 
+def gaussian_pulse_free_field(r, t, sigma, c0):
+    """Pressure at distance r and time t of a Gaussian pressure pulse exp(-|x|^2/(2 sigma^2)) released
+    at rest in free space (linear acoustics, v0 = 0): the spherical d'Alembert solution
+    p = [(r - c t) g(r - c t) + (r + c t) g(r + c t)] / (2 r) with g(s) = exp(-s^2/(2 sigma^2))."""
+    a, b = r - c0 * t, r + c0 * t
+    return (a * np.exp(-a * a / (2 * sigma ** 2)) + b * np.exp(-b * b / (2 * sigma ** 2))) / (2 * r)
+
+
+def image_source_series(src, rcv, L, sigma, c0, t, rigid=(True, True, True, True, True, False)):
+    """Receiver pressure series of the Gaussian pulse in the box [0,Lx]x[0,Ly]x[0,Lz] by images:
+    walls marked rigid (x=0, x=Lx, y=0, y=Ly, z=0, z=Lz) mirror the source with the same sign, the
+    others are taken as perfectly absorbing (no image). Obstacles inside the box are ignored.
+    src, rcv: (P, 3); t: (T,) times. Returns (P, T)."""
+    src = np.atleast_2d(np.asarray(src, np.float64))
+    rcv = np.atleast_2d(np.asarray(rcv, np.float64))
+    t = np.asarray(t, np.float64)
+    reach = c0 * t.max() + 8 * sigma
+    out = np.zeros((len(src), len(t)))
+
+    def axis_images(s, Lk, lo_rigid, hi_rigid, r):
+        """image coordinates along one axis within `reach` of the receivers r"""
+        if lo_rigid and hi_rigid:  # period 2L lattice: 2nL + s and 2nL - s
+            n = int(np.ceil(reach / (2 * Lk))) + 1
+            return [(2 * k * Lk + s) for k in range(-n, n + 1)] + [(2 * k * Lk - s) for k in range(-n, n + 1)]
+        pts = [s]
+        if lo_rigid:
+            pts.append(-s)
+        if hi_rigid:
+            pts.append(2 * Lk - s)
+        return pts
+
+    for i in range(len(src)):
+        imgs = [axis_images(src[i, k], L[k], rigid[2 * k], rigid[2 * k + 1], rcv[i, k]) for k in range(3)]
+        for x in imgs[0]:
+            dx = rcv[i, 0] - x
+            if abs(dx) > reach:
+                continue
+            for y in imgs[1]:
+                dy = rcv[i, 1] - y
+                if abs(dy) > reach:
+                    continue
+                for z in imgs[2]:
+                    r = np.sqrt(dx * dx + dy * dy + (rcv[i, 2] - z) ** 2)
+                    if r <= reach:
+                        out[i] += gaussian_pulse_free_field(max(r, 1e-9), t, sigma, c0)
+    return out
+
+
 def rel_l2(a, b):
     """Per-row relative L2 error |a - b| / |b|."""
     return np.linalg.norm(a - b, axis=-1) / np.maximum(np.linalg.norm(b, axis=-1), 1e-30)
@@ -172,14 +220,20 @@ def rel_l2(a, b):
 
 class Surrogate:
     """Two networks trained on the same data with different seeds. The answer is their mean; the
-    error estimate is their relative disagreement."""
+    error estimate is their relative disagreement. With a `base` model (a callable (src, rcv) ->
+    series, e.g. image_source_series), the networks learn the correction to it: answer = base +
+    mean network output."""
 
-    def __init__(self, lo, hi, out_dim, scale, seeds=(1, 2), **net_kw):
+    def __init__(self, lo, hi, out_dim, scale, seeds=(1, 2), base=None, **net_kw):
         self.lo = np.asarray(lo, np.float64)
         self.hi = np.asarray(hi, np.float64)
         self.scale = float(scale)
+        self.base = base
         self.cal = None  # scale of the disagreement that bounds the true error (calibrate())
         self.nets = [Net(out_dim=out_dim, seed=s, **net_kw) for s in seeds]
+
+    def _base(self, src, rcv, n):
+        return self.base(src, rcv) if self.base is not None else np.zeros((len(np.atleast_2d(src)), n))
 
     def encode(self, src, rcv):
         """Coordinates normalised to [0,1] by the domain box."""
@@ -191,10 +245,12 @@ class Surrogate:
         """Train every network on (src, rcv) -> series; the learning rate steps down through
         `decay` over equal thirds of the run."""
         x = self.encode(src, rcv)
-        t = np.asarray(series, np.float64) / self.scale
-        # relative L2 loss per sample; pairs the pulse has not reached within the window carry
-        # almost no signal, so their weight is capped at that of a series 1% of the largest norm
-        nrm2 = (t * t).sum(axis=1)
+        full = np.asarray(series, np.float64) / self.scale
+        t = full - self._base(src, rcv, full.shape[1]) / self.scale
+        # relative L2 loss per sample (relative to the full series); pairs the pulse has not reached
+        # within the window carry almost no signal, so their weight is capped at that of a series
+        # 1% of the largest norm
+        nrm2 = (full * full).sum(axis=1)
         w = 1.0 / np.maximum(nrm2, 1e-4 * nrm2.max())
         out = []
         for k, n in enumerate(self.nets):
@@ -212,6 +268,9 @@ class Surrogate:
         disagreement."""
         x = self.encode(src, rcv)
         ys = [n.forward(x).astype(np.float64) * self.scale for n in self.nets]
+        if self.base is not None:
+            b = self._base(src, rcv, ys[0].shape[1])
+            ys = [y + b for y in ys]
         mean = sum(ys) / len(ys)
         dis = rel_l2(ys[0], ys[1]) if len(ys) > 1 else np.zeros(len(mean))
         return mean, dis * (self.cal if self.cal is not None else 0.5)

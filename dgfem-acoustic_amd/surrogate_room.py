@@ -68,6 +68,7 @@ def main():
     ap.add_argument("--freq-sigma", type=float, default=0.1)
     ap.add_argument("--tol", type=float, default=0.02)
     ap.add_argument("--out", default="")
+    ap.add_argument("--dump", default="", help="write sources, receivers and DG series (float32) to this .npz")
     args = ap.parse_args()
     small = args.room == "small"
     S = args.sources or (64 if small else 40)
@@ -102,6 +103,9 @@ def main():
     series = surrogate.dg_impulse_responses(m, ph, dt, src, rcv, args.nsteps, sigma, stride, per_batch=12)
     t_data = time.perf_counter() - t0
     log(f"DG data: {S} sources x {R} receivers x {args.nsteps // stride} samples in {t_data:.1f} s")
+    if args.dump:
+        np.savez_compressed(args.dump, src=src, rcv=rcv, series=series.astype(np.float32), dt=dt, sigma=sigma,
+                            stride=stride, L=np.array(L), block_lo=block_box[0], block_hi=block_box[1])
     # one plain DG solve (single context, one source, all its receivers) for the per-query cost
     s = dgk.Solver(m, ph["c0"], ph["rho0"], precision=dgk.FP32, flux_sign=dgk.SIGN_STABLE)
     u = np.zeros((4, m.K * m.Np))

@@ -158,8 +158,10 @@ dgk_status dgk_batch_set_state(dgk_batch* b, int sim, const double* u);
 dgk_status dgk_batch_get_state(dgk_batch* b, int sim, double* u);
 /* schedule: 0 = groups as parallel graph branches, 1 = groups in sequence on one stream */
 dgk_status dgk_batch_rk4(dgk_batch* b, int nsteps, int schedule);
-/* receivers of member sim (node_xyz: that member's mesh); members of one group share a record
- * capacity (the largest requested) and setting any member's receivers clears the group's record */
+/* receivers of member sim (node_xyz: that member's mesh), recorded after every following step
+ * for at least `capacity` steps. Members of one group share one record: setting a member's
+ * receivers restarts that member's series (dgk_batch_get_receivers returns the steps since) and
+ * keeps the rows already recorded for the other members. */
 dgk_status dgk_batch_set_receivers(dgk_batch* b, int sim, int n, const double* xyz, const double* node_xyz,
                                    int capacity);
 dgk_status dgk_batch_get_receivers(dgk_batch* b, int sim, double* out, int capacity_rows, int* rows);

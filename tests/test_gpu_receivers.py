@@ -116,3 +116,28 @@ def test_batch_receivers_match_oracle():
         err = rel(rec, ref)
         print(f"  batch member {i}: receivers rel L2 {err:.2e}", flush=True)
         assert err <= 1e-12
+
+
+def test_batch_receivers_set_one_member_after_another():
+    """Members 1 and 3 share a group record. Member 1's receivers are set, 10 steps run, then
+    member 3's are set and 15 more steps run: member 1 keeps all 25 rows, member 3 records the 15
+    steps since its receivers were set."""
+    ms = [mixed_mesh(2, 3), mixed_mesh(3, 2), mixed_mesh(2, 3, 7), mixed_mesh(3, 2, 4)]
+    dts = [2e-6, 1.5e-6, 1e-6, 1.2e-6]
+    ph = dict(c0=340.0, rho0=1.225, v0=(0.0, 0.0, 0.0))
+    b = dgk.Batch(ms, [ph] * 4, dts, flux_sign=dgk.SIGN_STABLE)
+    assert len(b.groups()) == 2
+    u0 = [smooth_state(m, 5 + i) for i, m in enumerate(ms)]
+    for i, u in enumerate(u0):
+        b.set_state(i, u)
+    p1, p3 = points(3, 3, 3), points(3, 2, 4)
+    b.set_receivers(1, p1, 25)
+    b.rk4(10)
+    b.set_receivers(3, p3, 15)
+    b.rk4(15)
+    r1, r3 = b.get_receivers(1), b.get_receivers(3)
+    assert r1.shape == (25, 3) and r3.shape == (15, 2)
+    ref1 = oracle_series(ms[1], u0[1], 25, dts[1], p1)
+    ref3 = oracle_series(ms[3], u0[3], 25, dts[3], p3)[10:]
+    assert rel(r1, ref1) <= 1e-12
+    assert rel(r3, ref3) <= 1e-12
