@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -21,7 +22,14 @@
 
 #include "../../include/dgk_surrogate.h"
 
+// Scratch of the allocation-free forward pass (dgs_forward_noalloc): one copy per GPU, held until
+// the process ends. Host code never uses this name as an address; it uses the per-device pointer
+// that hipGetSymbolAddress returns (dgs::scratch_base).
+static __device__ __align__(256) unsigned char scratch[16u << 20];
+
 namespace dgs {
+
+constexpr size_t kScratchBytes = 16u << 20;
 
 namespace {
 
@@ -232,6 +240,7 @@ __global__ void k_gather(int n, int in, int N, const int* __restrict__ idx, cons
 // ------------------------------------------------------------------------------- the network
 struct Net {
   dgs_config cfg{};
+  int device = 0;                  // HIP device the network lives on
   int L = 0;                       // weight layers (depth hidden + output)
   std::vector<int> din, dout;      // per layer
   std::vector<int64_t> woff, boff; // offsets into the flat parameter vector
@@ -297,18 +306,78 @@ void gemm(Net& n, GemmArgs g) {
   DGS_HIP(hipGetLastError());
 }
 
+// Device buffers of one forward pass: inputs x [rows][in], features z, per hidden layer l the BF16
+// activations h(l) and FP32 pre-activations pre(l), output y [rows][out].
+struct Pass {
+  const float* x;
+  uint16_t* z;
+  uint16_t* h[2];  // hidden layer l writes h[l % 2] (when not per layer)
+  float* pre;      // one buffer for every hidden layer (when not per layer)
+  const std::vector<uint16_t*>* hl = nullptr;  // per-layer buffers (training keeps them for backward)
+  const std::vector<float*>* prel = nullptr;
+  float* y;
+};
+
+// rows [0, mp) of dst take source rows min(r, m - 1) of x (padding with copies of the last row)
+__global__ void k_pad_rows(const float* __restrict__ x, int m, int mp, int in, float* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= mp * in) return;
+  const int r = i / in, c = i - r * in;
+  dst[i] = x[(int64_t)min(r, m - 1) * in + c];
+}
+
+// Device address of `scratch` on device dev, looked up once per device with that device current.
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[64] = {};
+unsigned char* scratch_base(int dev) {
+  if (dev < 0 || dev >= 64) throw Error(DGS_ERR_INVALID, "device ordinal out of range for the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    int cur = 0;
+    DGS_HIP(hipGetDevice(&cur));
+    if (cur != dev) DGS_HIP(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    DGS_HIP(e);
+    g_scratch[dev] = (unsigned char*)p;
+  }
+  return g_scratch[dev];
+}
+
+// Layout of one forward pass of mp rows in the scratch array (256-byte aligned pieces); returns the
+// bytes needed.
+size_t scratch_layout(const Net& n, int mp, unsigned char* base, Pass* b) {
+  const dgs_config& k = n.cfg;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    unsigned char* p = base ? base + off : nullptr;
+    off += (bytes + 255) / 256 * 256;
+    return p;
+  };
+  float* x = (float*)take((size_t)mp * k.in_dim * sizeof(float));
+  b->x = x;
+  b->z = (uint16_t*)take((size_t)mp * 2 * k.n_freq * sizeof(uint16_t));
+  b->h[0] = (uint16_t*)take((size_t)mp * k.width * sizeof(uint16_t));
+  b->h[1] = (uint16_t*)take((size_t)mp * k.width * sizeof(uint16_t));
+  b->pre = (float*)take((size_t)mp * k.width * sizeof(float));
+  b->y = (float*)take((size_t)mp * k.out_dim * sizeof(float));
+  return off;
+}
+
 void check_batch(const Net& n, int rows) {
   if (rows <= 0 || rows > n.cfg.max_batch || rows % 64)
     throw Error(DGS_ERR_INVALID, "batch rows must be a positive multiple of 64 up to max_batch");
 }
 
 // forward pass of `rows` rows already in n.x; output in n.y
-void forward(Net& n, int rows) {
+void forward(Net& n, int rows);
+void forward(Net& n, int rows, const Pass& b) {
   const int F = n.cfg.n_freq, in = n.cfg.in_dim;
   const int64_t tot = (int64_t)rows * F;
-  hipLaunchKernelGGL(k_features, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, n.s, rows, in, F, n.x, n.Bm, n.z);
+  hipLaunchKernelGGL(k_features, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, n.s, rows, in, F, b.x, n.Bm, b.z);
   DGS_HIP(hipGetLastError());
-  const uint16_t* a = n.z;
+  const uint16_t* a = b.z;
   for (int l = 0; l < n.L; ++l) {
     GemmArgs g{};
     g.M = rows;
@@ -323,15 +392,25 @@ void forward(Net& n, int rows) {
     g.bias = n.P + n.boff[l];
     g.ldc = n.dout[l];
     if (l + 1 < n.L) {
-      g.pre = n.pre[l];
-      g.outb = n.h[l];
+      g.pre = b.prel ? (*b.prel)[l] : b.pre;
+      g.outb = b.hl ? (*b.hl)[l] : b.h[l % 2];
       gemm<false, false, EPI_BIAS_ACT>(n, g);
-      a = n.h[l];
+      a = g.outb;
     } else {
-      g.outf = n.y;
+      g.outf = b.y;
       gemm<false, false, EPI_BIAS>(n, g);
     }
   }
+}
+
+void forward(Net& n, int rows) {
+  Pass b{};
+  b.x = n.x;
+  b.z = n.z;
+  b.hl = &n.h;
+  b.prel = &n.pre;
+  b.y = n.y;
+  forward(n, rows, b);
 }
 
 // loss and gradients of the rows in (n.x, n.t, n.w); loss_row filled; gradients in n.G
@@ -399,6 +478,8 @@ void create_impl(Net& n, const dgs_config* c) {
       k.depth < 1 || k.out_dim < 64 || k.out_dim % 64 || k.max_batch < 64 || k.max_batch % 64)
     throw Error(DGS_ERR_INVALID, "unsupported network shape (width, out_dim, max_batch multiples of 64, n_freq of 32)");
   n.L = k.depth + 1;
+  DGS_HIP(hipGetDevice(&n.device));
+  (void)scratch_base(n.device);  // the allocation-free forward pass's scratch address on this device
   for (int l = 0; l < n.L; ++l) {
     n.din.push_back(l ? k.width : 2 * k.n_freq);
     n.dout.push_back(l + 1 < n.L ? k.width : k.out_dim);
@@ -521,6 +602,32 @@ dgs_status dgs_forward(dgs_net* n, int rows, const float* x, float* y) {
       DGS_HIP(hipMemcpyAsync(y + (size_t)r0 * No, n->y, (size_t)m * No * sizeof(float), hipMemcpyDeviceToHost, n->s));
       DGS_HIP(hipStreamSynchronize(n->s));
     }
+  });
+}
+
+dgs_status dgs_forward_noalloc(dgs_net* n, int rows, const float* x, float* y) {
+  if (!n || rows < 0 || (rows && (!x || !y))) return DGS_ERR_INVALID;
+  return dgs::guarded(n, [&] {
+    const int in = n->cfg.in_dim, No = n->cfg.out_dim;
+    dgs::Pass b{};
+    // rows per pass: the largest multiple of 64 (up to max_batch) whose buffers fit the scratch array
+    int mb = 0;
+    for (int r = 64; r <= n->cfg.max_batch; r += 64)
+      if (dgs::scratch_layout(*n, r, nullptr, &b) <= dgs::kScratchBytes) mb = r;
+    if (!mb)
+      throw dgs::Error(DGS_ERR_INVALID, "network too wide for the 16 MiB forward scratch (64 rows do not fit)");
+    unsigned char* base = dgs::scratch_base(n->device);
+    for (int r0 = 0; r0 < rows; r0 += mb) {
+      const int m = std::min(mb, rows - r0);
+      const int mp = (m + 63) / 64 * 64;  // whole 64-row tiles, padded with copies of the last row
+      dgs::scratch_layout(*n, mp, base, &b);
+      hipLaunchKernelGGL(dgs::k_pad_rows, dim3((unsigned)((mp * in + 255) / 256)), dim3(256), 0, n->s,
+                         x + (size_t)r0 * in, m, mp, in, (float*)b.x);
+      DGS_HIP(hipGetLastError());
+      dgs::forward(*n, mp, b);
+      DGS_HIP(hipMemcpyAsync(y + (size_t)r0 * No, b.y, (size_t)m * No * sizeof(float), hipMemcpyDeviceToDevice, n->s));
+    }
+    DGS_HIP(hipStreamSynchronize(n->s));
   });
 }
 

@@ -40,6 +40,7 @@ def _lib():
         L.dgs_set_params.argtypes = [vp, fp, fp]
         L.dgs_forward.argtypes = [vp, ctypes.c_int, fp, fp]
         L.dgs_forward_timed.argtypes = [vp, ctypes.c_int, ctypes.c_int, _P(ctypes.c_double)]
+        L.dgs_forward_noalloc.argtypes = [vp, ctypes.c_int, vp, vp]
         L.dgs_gradients.argtypes = [vp, ctypes.c_int, fp, fp, fp, fp, _P(ctypes.c_double)]
         L.dgs_set_data.argtypes = [vp, ctypes.c_int64, fp, fp, fp]
         L.dgs_train.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, fp]
@@ -48,7 +49,7 @@ def _lib():
 
 
 SYMBOLS = ("dgs_create", "dgs_destroy", "dgs_last_error", "dgs_num_params", "dgs_get_params", "dgs_set_params",
-           "dgs_forward", "dgs_forward_timed", "dgs_gradients", "dgs_set_data", "dgs_train")
+           "dgs_forward", "dgs_forward_timed", "dgs_gradients", "dgs_set_data", "dgs_train", "dgs_forward_noalloc")
 
 
 def _f(a):
@@ -106,6 +107,12 @@ class Net:
         y = np.zeros((len(xa), self.cfg["out_dim"]), np.float32)
         self._check(_lib().dgs_forward(self.h, len(xa), xp, y.ctypes.data_as(_P(ctypes.c_float))), "dgs_forward")
         return y
+
+    def forward_device(self, rows, x_ptr, y_ptr):
+        """Forward pass on device buffers (addresses x_ptr [rows][in_dim], y_ptr [rows][out_dim], FP32)
+        without any device allocation (dgs_forward_noalloc)."""
+        self._check(_lib().dgs_forward_noalloc(self.h, int(rows), ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr)),
+                    "dgs_forward_noalloc")
 
     def forward_ms(self, rows, reps=20):
         ms = ctypes.c_double()

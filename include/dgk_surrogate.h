@@ -43,6 +43,14 @@ dgs_status dgs_get_params(dgs_net* net, float* params, float* fourier);
 dgs_status dgs_set_params(dgs_net* net, const float* params, const float* fourier);
 /* y[rows][out_dim] = net(x[rows][in_dim]) (host buffers, any row count) */
 dgs_status dgs_forward(dgs_net* net, int rows, const float* x, float* y);
+/* y[rows][out_dim] = net(x[rows][in_dim]) with x and y in device memory of the network's GPU.
+ * No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops): it calls no allocating routine; its activations live
+ * in a fixed 16 MiB device array per GPU (rows are processed in passes of the largest multiple
+ * of 64 whose buffers fit; a network whose 64-row pass does not fit is rejected with
+ * DGS_ERR_INVALID). Ordered on the network's stream and synchronous like dgs_forward. Calls
+ * that overlap on one GPU share that array and race. */
+dgs_status dgs_forward_noalloc(dgs_net* net, int rows, const float* x, float* y);
 /* average device time of `reps` forward passes over `rows` rows already on the device */
 dgs_status dgs_forward_timed(dgs_net* net, int rows, int reps, double* ms_per_rep);
 /* one forward + backward without update: loss = sum_r w[r] |y_r - t_r|^2 and its gradient
