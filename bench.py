@@ -323,7 +323,7 @@ def smallest_member_per_shape(n=256):
     return sorted(i for _, i in best.values())
 
 
-def cpu_baseline(meshes, steps=1):
+def cpu_baseline(meshes, steps=6):
     """The oracle on a bounded sample of the ensemble, all host threads: `steps` RK4 steps of one
     member per (dim, order) -- the smallest of its shape, drawn again with about 3000 (2D), 1500
     (3D orders 2-3) or 600 (3D orders 4-5) elements (same seed, shape, sides, walls and pulse), about
@@ -373,7 +373,7 @@ def cpu_baseline(meshes, steps=1):
                           for (d, p), r in sorted(rates.items())}}
 
 
-def cube_cpu(steps=2, n=12):
+def cube_cpu(steps=8, n=12):
     """The oracle beside the FP64 cube leg: `steps` RK4 steps of the same generator at n^3 cells
     (the full 32^3 cube takes about a minute per step on the literal path), literal and factorized
     paths, all host threads."""

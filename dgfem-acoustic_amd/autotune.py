@@ -14,6 +14,8 @@ dgk.py loads with the library. Results never depend on the launch shape; the tes
 that (tests/test_gpu_tuning.py).
 
 Usage: python dgfem-acoustic_amd/autotune.py [--dofs 4e6] [--out PATH] [--dims 2,3] [--orders 1-5]
+       python dgfem-acoustic_amd/autotune.py --time-only [--steps 5] [--dims 3] [--orders 4,5]
+(--time-only: time the shipped table's choice on each shape, for profiling runs; writes nothing)
 """
 import argparse
 import os
@@ -72,7 +74,29 @@ def main():
     ap.add_argument("--precisions", default="fp64,fp32")
     ap.add_argument("--no-lds", action="store_true", help="skip the thread-per-node variant")
     ap.add_argument("--out", default=os.path.join(HERE, "tuning.txt"))
+    ap.add_argument("--time-only", action="store_true",
+                    help="run `--steps` RK4 steps of each shape with the shipped tuning table's launch settings "
+                         "(for counter runs) and print the stage time")
+    ap.add_argument("--steps", type=int, default=5)
     args = ap.parse_args()
+    if args.time_only:
+        L = dgk.device()
+        if L.dgk_load_tuning(dgk.TUNING_FILE.encode()):
+            raise RuntimeError(L.dgk_last_error(None).decode())
+        precs = {"fp64": dgk.FP64, "fp32": dgk.FP32}
+        for dim in [int(d) for d in args.dims.split(",")]:
+            for p in [int(o) for o in args.orders.split(",")]:
+                m = mesh_for(dim, p, args.dofs)
+                for pname in args.precisions.split(","):
+                    s = dgk.Solver(m, 340.0, 1.225, precision=precs[pname], flux_sign=dgk.SIGN_STABLE)
+                    u = np.zeros((4, m.K * m.Np))
+                    u[0] = np.exp(-((m.node_xyz - m.node_xyz.mean(axis=0)) ** 2).sum(axis=1) / 0.01)
+                    s.set_state(u)
+                    ms, _ = s.rk4_timed(args.steps, 1e-7)
+                    print(f"[autotune] time-only dim={dim} p={p} {pname} K={m.K}: {ms * 1e3 / (4 * args.steps):.1f} us "
+                          f"per stage", flush=True)
+                    s.close()
+        return
     lines = ["# dim order precision variant grid_pct tile_order stage_us  (written by autotune.py on %s)"
              % time.strftime("%Y-%m-%d")]
     precs = {"fp64": dgk.FP64, "fp32": dgk.FP32}

@@ -191,32 +191,26 @@ def image_source_series(src, rcv, L, sigma, c0, t, rigid=(True, True, True, True
     reach = c0 * t.max() + 8 * sigma
     out = np.zeros((len(src), len(t)))
 
-    def axis_images(s, Lk, lo_rigid, hi_rigid, r):
-        """image coordinates along one axis within `reach` of the receivers r"""
+    def axis_images(s, Lk, lo_rigid, hi_rigid):
+        """image coordinates along one axis (within `reach` of the box)"""
         if lo_rigid and hi_rigid:  # period 2L lattice: 2nL + s and 2nL - s
             n = int(np.ceil(reach / (2 * Lk))) + 1
-            return [(2 * k * Lk + s) for k in range(-n, n + 1)] + [(2 * k * Lk - s) for k in range(-n, n + 1)]
+            k = np.arange(-n, n + 1)
+            return np.concatenate([2 * k * Lk + s, 2 * k * Lk - s])
         pts = [s]
         if lo_rigid:
             pts.append(-s)
         if hi_rigid:
             pts.append(2 * Lk - s)
-        return pts
+        return np.array(pts)
 
     for i in range(len(src)):
-        imgs = [axis_images(src[i, k], L[k], rigid[2 * k], rigid[2 * k + 1], rcv[i, k]) for k in range(3)]
-        for x in imgs[0]:
-            dx = rcv[i, 0] - x
-            if abs(dx) > reach:
-                continue
-            for y in imgs[1]:
-                dy = rcv[i, 1] - y
-                if abs(dy) > reach:
-                    continue
-                for z in imgs[2]:
-                    r = np.sqrt(dx * dx + dy * dy + (rcv[i, 2] - z) ** 2)
-                    if r <= reach:
-                        out[i] += gaussian_pulse_free_field(max(r, 1e-9), t, sigma, c0)
+        ax = [axis_images(src[i, k], L[k], rigid[2 * k], rigid[2 * k + 1]) for k in range(3)]
+        d = [rcv[i, k] - ax[k] for k in range(3)]
+        d = [dk[np.abs(dk) <= reach] for dk in d]
+        r = np.sqrt(d[0][:, None, None] ** 2 + d[1][None, :, None] ** 2 + d[2][None, None, :] ** 2).ravel()
+        r = np.maximum(r[r <= reach], 1e-9)
+        out[i] = gaussian_pulse_free_field(r[:, None], t[None, :], sigma, c0).sum(axis=0)
     return out
 
 
