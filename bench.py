@@ -189,17 +189,8 @@ def batch_leg(steps, warmup, members_checked=8, n=256):
     log(f"batch: device setup and state upload {time.perf_counter() - t0:.1f} s, {len(groups)} groups")
     b.rk4(warmup)
     # schedule: groups as parallel graph branches or in sequence, whichever runs a short
-    # captured trial faster on this GPU
-    trial = {}
-    for sched in (dgk.SCHEDULE_PARALLEL, dgk.SCHEDULE_SEQUENTIAL):
-        b.prepare(10, sched)
-        b.sync()
-        t0 = time.perf_counter()
-        b.rk4(10, sched)
-        b.sync()
-        trial[sched] = time.perf_counter() - t0
-    # parallel branches make the per-kernel trace durations overlap; take them only when clearly faster
-    schedule = dgk.SCHEDULE_PARALLEL if trial[0] < 0.97 * trial[1] else dgk.SCHEDULE_SEQUENTIAL
+    # captured trial faster on this GPU (10 steps each)
+    schedule, trial = b.pick_schedule(10)
     log(f"batch: schedule trial (10 steps) parallel {trial[0]:.3f} s, sequential {trial[1]:.3f} s")
     total_before = warmup + 20
     t0 = time.perf_counter()

@@ -1,17 +1,22 @@
-"""Autotuner for the stage kernel's launch shape and the batch schedule.
+"""Autotuner for the stage kernel's tile shape and launch shape (the batch schedule is picked
+per ensemble by dgk.Batch.pick_schedule).
 
 For each (dim, order, precision) it builds a representative mesh (about `dofs` unknowns: a
 structured 2D rectangle or a 3D Kuhn box), times every candidate with HIP events around each
 stage launch (dgk_rk4_timed) and keeps the fastest:
 
   variant     thread-per-node kernel (1, FP64 only) or MFMA kernel (2)
+  tile_shape  the MFMA kernel's workgroup and tile (dgk_set_tile_shape): 0 built-in, 1 4 waves x 16
+              elements, 2 8 waves x 32 elements, 3 row split (4 waves x 4 elements), 4 4 waves x 16
+              elements with the operator in global memory; shapes without a kernel for the case
+              are skipped
   grid_pct    persistent grid as a percentage of the occupancy-derived grid: 50, 100, 200, 400
   tile_order  XCD-aware slabs (0) or round-robin (1)
 
-The table is written as text lines "dim order precision variant grid_pct tile_order stage_us"
-(libdgk reads the first six columns, dgk_load_tuning) to dgfem-acoustic_amd/tuning.txt, which
-dgk.py loads with the library. Results never depend on the launch shape; the test suite checks
-that (tests/test_gpu_tuning.py).
+The table is written as text lines "dim order precision variant grid_pct tile_order tile_shape
+stage_us" (dgk_load_tuning) to dgfem-acoustic_amd/tuning.txt, which dgk.py loads with the
+library. Results never depend on the launch shape and agree with the built-in tile shape to
+rounding; the test suite checks both (tests/test_gpu_tuning.py).
 
 Usage: python dgfem-acoustic_amd/autotune.py [--dofs 4e6] [--out PATH] [--dims 2,3] [--orders 1-5]
        python dgfem-acoustic_amd/autotune.py --time-only [--steps 5] [--dims 3] [--orders 4,5]
@@ -51,17 +56,27 @@ def mesh_for(dim, p, dofs):
 
 
 def time_case(m, precision, variant, nsteps=10):
+    """Stage time (us) of every candidate, keyed (variant, grid_pct, tile_order, tile_shape)."""
     s = dgk.Solver(m, 340.0, 1.225, precision=precision, flux_sign=dgk.SIGN_STABLE, variant=variant)
     u = np.zeros((4, m.K * m.Np))
     u[0] = np.exp(-((m.node_xyz - m.node_xyz.mean(axis=0)) ** 2).sum(axis=1) / 0.01)
     s.set_state(u)
     res = {}
-    for g in GRID if variant == dgk.VARIANT_MFMA else (100,):
-        for o in ORDERS if variant == dgk.VARIANT_MFMA else (0,):
-            s.set_launch(g, o)
-            s.rk4_timed(2, 1e-7)  # warm the launch shape
-            ms, _ = s.rk4_timed(nsteps, 1e-7)
-            res[(variant, g, o)] = ms * 1e3 / (4 * nsteps)
+    mfma = variant == dgk.VARIANT_MFMA
+    for sh in dgk.TILE_SHAPES if mfma else (0,):
+        if mfma:
+            try:
+                s.set_tile_shape(sh)
+            except dgk.DGKError as e:
+                if e.status != dgk.DGK_ERR_UNSUPPORTED:
+                    raise
+                continue  # no kernel of this shape for the case
+        for g in GRID if mfma else (100,):
+            for o in ORDERS if mfma else (0,):
+                s.set_launch(g, o)
+                s.rk4_timed(2, 1e-7)  # warm the launch shape
+                ms, _ = s.rk4_timed(nsteps, 1e-7)
+                res[(variant, g, o, sh)] = ms * 1e3 / (4 * nsteps)
     s.close()
     return res
 
@@ -84,6 +99,11 @@ def main():
         if L.dgk_load_tuning(dgk.TUNING_FILE.encode()):
             raise RuntimeError(L.dgk_last_error(None).decode())
         precs = {"fp64": dgk.FP64, "fp32": dgk.FP32}
+        table = {}
+        for line in open(dgk.TUNING_FILE):
+            w = line.split()
+            if w and not w[0].startswith("#"):
+                table[(int(w[0]), int(w[1]), int(w[2]))] = w[3:]
         for dim in [int(d) for d in args.dims.split(",")]:
             for p in [int(o) for o in args.orders.split(",")]:
                 m = mesh_for(dim, p, args.dofs)
@@ -93,11 +113,12 @@ def main():
                     u[0] = np.exp(-((m.node_xyz - m.node_xyz.mean(axis=0)) ** 2).sum(axis=1) / 0.01)
                     s.set_state(u)
                     ms, _ = s.rk4_timed(args.steps, 1e-7)
+                    row = table.get((dim, p, precs[pname]))
                     print(f"[autotune] time-only dim={dim} p={p} {pname} K={m.K}: {ms * 1e3 / (4 * args.steps):.1f} us "
-                          f"per stage", flush=True)
+                          f"per stage (table row: {' '.join(row) if row else 'none'})", flush=True)
                     s.close()
         return
-    lines = ["# dim order precision variant grid_pct tile_order stage_us  (written by autotune.py on %s)"
+    lines = ["# dim order precision variant grid_pct tile_order tile_shape stage_us  (written by autotune.py on %s)"
              % time.strftime("%Y-%m-%d")]
     precs = {"fp64": dgk.FP64, "fp32": dgk.FP32}
     for dim in [int(d) for d in args.dims.split(",")]:
@@ -110,11 +131,15 @@ def main():
                 # the default FP32 form (compensated solution) stays on the MFMA kernel
                 if not args.no_lds and prec == dgk.FP64:
                     res.update(time_case(m, prec, dgk.VARIANT_LDS))
-                (v, g, o), us = min(res.items(), key=lambda kv: kv[1])
-                base = res[(dgk.VARIANT_MFMA, 100, 0)]
-                print(f"[autotune] dim={dim} p={p} {pname} K={m.K}: best variant={v} grid={g}% order={o} "
-                      f"{us:.1f} us (default {base:.1f} us, {base / us:.2f}x)", file=sys.stderr, flush=True)
-                lines.append(f"{dim} {p} {prec} {v} {g} {o} {us:.2f}")
+                (v, g, o, sh), us = min(res.items(), key=lambda kv: kv[1])
+                base = res[(dgk.VARIANT_MFMA, 100, 0, 0)]
+                shapes = {k[3]: min(t for kk, t in res.items() if kk[0] == dgk.VARIANT_MFMA and kk[3] == k[3])
+                          for k in res if k[0] == dgk.VARIANT_MFMA}
+                print(f"[autotune] dim={dim} p={p} {pname} K={m.K}: best variant={v} shape={sh} grid={g}% order={o} "
+                      f"{us:.1f} us (default {base:.1f} us, {base / us:.2f}x; best per tile shape: "
+                      + ", ".join(f"{k}: {t:.1f}" for k, t in sorted(shapes.items())) + ")", file=sys.stderr,
+                      flush=True)
+                lines.append(f"{dim} {p} {prec} {v} {g} {o} {sh} {us:.2f}")
     with open(args.out, "w") as f:
         f.write("\n".join(lines) + "\n")
     print(args.out)

@@ -346,9 +346,12 @@ void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* 
     }
   c.Fb = nbnd;
   c.F = (K * c.Nf + nbnd) / 2;
+  // the tuning table's tile shape, where this context's form has a kernel of that shape
+  if (have_tuning && tuned.tile_shape != 0 && c.variant == kVariantMfma) {
+    StageArgs a = base_args(c);
+    if (tile_shape_supported(c, a, tuned.tile_shape)) c.tile_shape = tuned.tile_shape;
+  }
 }
-
-namespace {
 
 StageArgs base_args(Ctx& c) {
   StageArgs a{};
@@ -380,8 +383,6 @@ StageArgs base_args(Ctx& c) {
   for (int k = 0; k < 3; ++k) a.v0[k] = c.v0[k];
   return a;
 }
-
-}  // namespace
 
 // Launch the four RK4 stages of one step on stream s.
 void rk4_step(Ctx& c, double dt, hipStream_t s, hipEvent_t* ev) {
@@ -853,6 +854,31 @@ dgk_status dgk_set_launch(dgk_ctx* c, int grid_pct, int tile_order) {
   });
 }
 
+dgk_status dgk_stage_addressing(int dim, int order, int precision, int64_t num_elements, int* addr64) {
+  if (!addr64 || dim < 1 || dim > 3 || order < 1 || order > 5 || num_elements <= 0 ||
+      (precision != DGK_FP64 && precision != DGK_FP32))
+    return fail(nullptr, DGK_ERR_INVALID, "bad addressing query");
+  if ((num_elements + kTilePad) * (dim + 1) >= ((int64_t)1 << 31))
+    return fail(nullptr, DGK_ERR_UNSUPPORTED, "more than 2^31 element faces");
+  int64_t np = 1;
+  for (int k = 1; k <= dim; ++k) np = np * (order + k) / k;  // C(order + dim, dim)
+  const int64_t kpad = (num_elements + kTilePad - 1) / kTilePad * kTilePad;
+  *addr64 = stage_needs_addr64(kpad, (int)np, dim + 1, dim == 3 ? 4 : dim + 1, precision == DGK_FP64 ? 8 : 4);
+  return DGK_OK;
+}
+
+dgk_status dgk_set_tile_shape(dgk_ctx* c, int shape) {
+  if (!c || shape < 0 || shape >= kTileShapes) return fail(c, DGK_ERR_INVALID, "bad tile shape");
+  return guarded(c, [&] {
+    StageArgs a = base_args(*c);
+    if (!tile_shape_supported(*c, a, shape))
+      throw Unsupported("tile shape " + std::to_string(shape) + " has no kernel for this context's form");
+    DGK_HIP(hipStreamSynchronize(c->stream));
+    c->tile_shape = shape;
+    c->drop_graphs();
+  });
+}
+
 dgk_status dgk_load_tuning(const char* path) {
   if (!path) return fail(nullptr, DGK_ERR_INVALID, "NULL path");
   return guarded(nullptr, [&] {
@@ -867,9 +893,22 @@ dgk_status dgk_load_tuning(const char* path) {
       Tuned v;
       if (!(ss >> d >> p >> prec >> v.variant >> v.grid_pct >> v.tile_order))
         throw InvalidArg("malformed tuning line: " + line);
+      // then either "tile_shape stage_us" or, in tables without the shape column, "stage_us"
+      std::vector<std::string> rest;
+      for (std::string w; ss >> w;) rest.push_back(w);
+      if (rest.size() > 2) throw InvalidArg("malformed tuning line: " + line);
+      if (rest.size() == 2) {
+        size_t used = 0;
+        try {
+          v.tile_shape = std::stoi(rest[0], &used);
+        } catch (const std::exception&) {
+          used = 0;
+        }
+        if (used != rest[0].size()) throw InvalidArg("malformed tuning line: " + line);
+      }
       if (d < 1 || d > 3 || p < 1 || p > 5 || (prec != DGK_FP64 && prec != DGK_FP32) ||
           (v.variant != kVariantLds && v.variant != kVariantMfma) || v.grid_pct < 1 || v.grid_pct > 10000 ||
-          v.tile_order < 0 || v.tile_order > 1)
+          v.tile_order < 0 || v.tile_order > 1 || v.tile_shape < 0 || v.tile_shape >= kTileShapes)
         throw InvalidArg("out-of-range tuning line: " + line);
       t[{d, p, prec}] = v;
     }
@@ -1348,6 +1387,20 @@ dgk_status dgk_batch_set_launch(dgk_batch* b, int group, int grid_pct, int tile_
     dgk::Ctx& c = *b->groups[group].ctx;
     c.grid_pct = grid_pct;
     c.tile_order = tile_order;
+    b->drop_graphs();
+  });
+}
+
+dgk_status dgk_batch_set_tile_shape(dgk_batch* b, int group, int shape) {
+  if (!b || group < 0 || group >= (int)b->groups.size() || shape < 0 || shape >= dgk::kTileShapes)
+    return bfail(b, DGK_ERR_INVALID, "bad tile shape");
+  return bguard(b, [&] {
+    dgk::Ctx& c = *b->groups[group].ctx;
+    dgk::StageArgs a = dgk::base_args(c);
+    if (!dgk::tile_shape_supported(c, a, shape))
+      throw dgk::Unsupported("tile shape " + std::to_string(shape) + " has no kernel for this group's form");
+    DGK_HIP(hipStreamSynchronize(b->main));
+    c.tile_shape = shape;
     b->drop_graphs();
   });
 }

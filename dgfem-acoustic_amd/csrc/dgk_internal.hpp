@@ -37,6 +37,17 @@ constexpr int kTilePad = 32;  // device arrays hold a multiple of this many elem
 // stage-kernel variants (dgk_opts.variant; 0 picks the default)
 constexpr int kVariantLds = 1;   // thread per element node, operators read from memory
 constexpr int kVariantMfma = 2;  // 16x16x4 MFMA contraction, one wave per 4 elements
+// The MFMA stage kernel addresses its arrays with 32-bit byte offsets from the buffers' own bases
+// while every one of them (state: Kpad*np*es values, J^-1: Kpad*9, face geometry: Kpad*nf*4) stays
+// below 4 GiB, and through 64-bit per-tile bases otherwise.
+inline bool stage_needs_addr64(int64_t Kpad, int np, int nf, int es, int tsize) {
+  const uint64_t lim = (uint64_t)1 << 32, kp = (uint64_t)Kpad;
+  return kp * np * es * tsize >= lim || kp * 9 * tsize >= lim || kp * nf * 4 * tsize >= lim;
+}
+// MFMA stage-kernel tile shapes (dgk_set_tile_shape): 0 built-in, 1 4 waves x 16 elements,
+// 2 8 waves x 32 elements, 3 row split (4 waves x 4 elements), 4 4 waves x 16 elements with the
+// operator read from global memory
+constexpr int kTileShapes = 5;
 constexpr int kMaxNFP = 21;
 constexpr int kMaxPerm = 6;
 
@@ -104,9 +115,9 @@ struct Ctx {
   // components), so their planes are neither read nor written: nq = dim+1. A state with nonzero
   // values there keeps all 4 planes live.
   int nq = 4;
-  // launch shape (tunable): persistent grid as a percentage of the occupancy-derived grid, and
-  // the tile order
-  int grid_pct = 100, tile_order = 0;
+  // launch shape (tunable): persistent grid as a percentage of the occupancy-derived grid, the
+  // tile order, and the MFMA kernel's tile shape (kTileShapes; 0 the built-in one)
+  int grid_pct = 100, tile_order = 0, tile_shape = 0;
   bool addr64 = false;  // force the 64-bit addressing form of the stage kernel
   int32_t *nbr = nullptr, *code = nullptr, *nface = nullptr, *perm = nullptr;
   double* stage_d = nullptr;  // FP64 staging for state transfer in FP32 mode
@@ -161,6 +172,9 @@ void build_geometry(Ctx& c, const double* vxyz_dev);
 void scale_geometry(Ctx& c, int64_t e0, int64_t n, double s);
 // stage.hip
 void launch_stage(Ctx& c, StageArgs& a, hipStream_t s);
+// whether the MFMA kernel has a kernel of tile shape `shape` for the context as it is (form, live
+// fields, buffer sizes); `a` as for launch_stage, nothing is launched
+bool tile_shape_supported(Ctx& c, StageArgs& a, int shape);
 void launch_source(Ctx& c, const Ctx::Src& s, double value, hipStream_t st);
 // hard source overwrite at the step whose start time is c.tdev[*c.tstep] (when t < duration)
 void launch_source_timed(Ctx& c, const Ctx::Src& s, hipStream_t st);
@@ -179,11 +193,12 @@ void mfma_shape(int dim, int p, bool fp32, bool oplo, int* KS, int* MB, int* NT)
 extern thread_local std::string g_err;
 void create_impl(Ctx& c, const dgk_mesh* m, const dgk_phys* ph, const dgk_opts* o);
 void rk4_step(Ctx& c, double dt, hipStream_t s, hipEvent_t* ev = nullptr);
+StageArgs base_args(Ctx& c);  // stage-kernel arguments over the context's buffers
 double* staging(Ctx& c);
 int live_planes(int dim, const double* u, int64_t n);  // nq of a host state u[4][n]
 // tuning table: launch settings per (dim, order, precision), loaded by dgk_load_tuning
 struct Tuned {
-  int variant = 0, grid_pct = 100, tile_order = 0;
+  int variant = 0, grid_pct = 100, tile_order = 0, tile_shape = 0;
 };
 bool find_tuning(int dim, int p, int precision, Tuned* t);
 struct RefElem;

@@ -321,22 +321,45 @@ __device__ __forceinline__ uint32_t f32_to_bf16_bits(float f) {
 // FP64 order 5 (129 KB operator) splits rows instead of elements: a 4-wave workgroup owns a tile of
 // 4 elements (21 KB of tile buffers beside the operator in LDS) and wave w computes row block w of
 // all 16 columns, so the operator never leaves LDS; every wave builds the same B operand.
-template <typename T, int D, int P>
+// Tile shapes of the stage kernel (dgk_set_tile_shape): 0 the built-in choice per (precision,
+// dim, order) below; 1 4-wave workgroups over 16-element tiles; 2 8-wave workgroups over
+// 32-element tiles; 3 the row split, 4 waves over a 4-element tile with wave w computing a quarter
+// of the operator's 16-row blocks (the whole operator in LDS); 4 4-wave workgroups over
+// 16-element tiles with the operator fragments read from global memory.
+// Built-in: 8-wave workgroups for 3D orders 4 and 5 (two waves per SIMD share one LDS copy of the
+// operator), the row split for FP64 3D order 5 (its 129 KB operator fits LDS only beside a small
+// tile), 4-wave workgroups elsewhere.
+template <typename T, int D, int P, int SH = 0>
 struct MfmaLaunch {
-  static constexpr bool ROWSPLIT = sizeof(T) == 8 && D == 3 && P == 5;
-  static constexpr int WAVES = (D == 3 && (P == 5 || P == 4) && !ROWSPLIT) ? 8 : 4;
+  static constexpr bool DEF_ROWSPLIT = sizeof(T) == 8 && D == 3 && P == 5;
+  static constexpr int DEF_WAVES = (D == 3 && (P == 5 || P == 4) && !DEF_ROWSPLIT) ? 8 : 4;
+  static constexpr bool ROWSPLIT = SH == 3 || (SH == 0 && DEF_ROWSPLIT);
+  static constexpr int WAVES = SH == 0 ? DEF_WAVES : SH == 2 ? 8 : 4;
+  static constexpr bool OPG = SH == 4;                // operator fragments from global memory
   static constexpr int KB = 64 * WAVES;              // threads per workgroup
   static constexpr int TE = ROWSPLIT ? 4 : 4 * WAVES;  // elements per tile
   static constexpr int RG = WAVES / (TE / 4);        // row groups: waves sharing one element group
 };
 
-// Pad (in values) of the LDS element stride NP*ES: FP32 to 4 or 12 mod 16 words, FP64 to 32 mod 64
-// words (the 4 elements of a wave then use different banks); none for the 2- and 3-field layouts
+// Pad (in values) of the LDS element stride NP*ES. A wave's 4 elements read and write node rows at
+// the same offset, 16 lanes (one lane group) or 32 lanes per LDS pass: FP32 strides of 4 or 12 mod
+// 16 words, FP64 strides of 16 or 48 mod 64 words (8 mod 16 doubles) put them in disjoint banks.
+// None for the 2- and 3-field layouts.
 template <typename T, int ES>
 constexpr int lds_pad(int np) {
   if (ES != 4) return 0;
   if (sizeof(T) == 4) return np % 4 == 0 || np % 4 == 2 ? 4 : 0;
-  return ((32 - 8 * (np % 8)) % 64 + 64) % 64 / 2;
+  return ((8 - 4 * np) % 16 + 16) % 16;
+}
+
+// LDS row stride of the face fluxes [element][field][row]: a face k-step's 64 lanes read value
+// c*RB + g + const (column c < 16, lane group g < 4). 4-byte values: a stride of 4 x odd puts the
+// 64 lanes in 64 distinct banks; 8-byte values: 2 x odd puts the 16 columns 4 words apart, so each
+// pass of one or two lane groups is conflict-free.
+template <typename T>
+constexpr int rb_stride(int rb) {
+  const int m = sizeof(T) == 4 ? 8 : 4, r = sizeof(T) == 4 ? 4 : 2;
+  return rb + ((r - rb % m) % m + m) % m;
 }
 
 // ES consecutive field values of one node (ES = 4: one vector read), zero beyond ES
@@ -353,25 +376,58 @@ __device__ __forceinline__ void node_fields(const T* p, T f[4]) {
   }
 }
 
-template <typename T, int D, int P, bool LO, bool CS, int ES, bool BIG>
-__global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageArgs a) {
+// Footprint of one stage-kernel instantiation: tile, operator and LDS sizes (shared by the kernel
+// and the host, which builds only the tile shapes that fit).
+template <typename T, int D, int P, bool LO, int ES, int SH>
+struct MfmaCfg {
+  using S = Sz<D, P>;
+  using M = MfmaShape<D, P>;
+  using L = MfmaLaunch<T, D, P, SH>;
+  static constexpr int NP = S::NP, NF = S::NF, NFP = S::NFP, NPERM = S::NPERM;
+  static constexpr int NT4 = M::tail(sizeof(T) == 4, LO);  // 4-row tail pieces (0: all blocks 16x16x4)
+  static constexpr int MBF = NT4 ? M::MB - 1 : M::MB;      // 16x16x4 row blocks
+  // A fragments per k-step: MBF 16x16x4 blocks of 64 lane values, and the tail piece of 16
+  // distinct values (row l%4 at lane group l/16; lanes 4 apart in l/4 read the same value)
+  static constexpr int NAV = (MBF * 64 + NT4 * 16) * M::KS;  // operator values
+  static constexpr int TE = L::TE, RG = L::RG;
+  // LDS element stride LES of the state tile and of the results: NP*ES values plus a pad (4-field
+  // layouts only, whose element boundaries fall on whole 16-byte vectors) chosen so that the four
+  // elements of a wave fall in different LDS banks. A volume k-step's lanes read node 4m+g of their
+  // element (16 or 32 bytes) and the result write-out puts row 4g+j (FP32) or g+4j (FP64) of 4
+  // elements x 4 fields: FP32 wants the stride in 4-byte words = 4 mod 16 (or 12), FP64 32 mod 64.
+  static constexpr int PADV = lds_pad<T, ES>(NP);
+  static constexpr int LES = NP * ES + PADV;
+  // face fluxes [element][field][RBP], then the results in the state tile's layout
+  static constexpr int RBP = rb_stride<T>(M::RB);
+  static constexpr int NSB = 4 * TE * RBP > TE * LES ? 4 * TE * RBP : TE * LES;
+  static constexpr size_t kLdsRest = sizeof(T) * (TE * LES + NSB + TE * NF * 4 + TE * 9) +
+                                     sizeof(int) * (NF * NPERM * NFP + NF * NFP) + (LO ? 16 * M::MB * M::NLO * 64 : 0);
+  static constexpr size_t kLdsMax = 160 * 1024 - 512;
+  // Operator fragments are staged in LDS whenever the whole workgroup footprint fits the 160 KiB
+  // of a gfx950 CU (fewer resident workgroups, but no per-k-step global loads).
+  static constexpr bool kALds = !L::OPG && NAV * sizeof(T) + kLdsRest <= kLdsMax;
+  // the row split needs whole 16-row blocks per wave and the operator in LDS; the 8-wave tile shape
+  // exists to share one LDS copy of the operator between two waves per SIMD (the built-in shape of
+  // the FP32 3D order-5 operator correction runs 8 waves with the operator in global memory)
+  static constexpr bool fits = kLdsRest <= kLdsMax && (RG == 1 || (NT4 == 0 && !LO && MBF % RG == 0 && kALds)) &&
+                               (SH == 0 || L::WAVES == 4 || kALds);
+};
+
+template <typename T, int D, int P, bool LO, bool CS, int ES, bool BIG, int SH>
+__global__ __launch_bounds__((MfmaLaunch<T, D, P, SH>::KB)) void k_stage_mfma(StageArgs a) {
   static_assert(!CS || sizeof(T) == 4, "the compensated state is an FP32 form");
   using S = Sz<D, P>;
   using M = MfmaShape<D, P>;
-  constexpr int kBlock = MfmaLaunch<T, D, P>::KB;
+  using C = MfmaCfg<T, D, P, LO, ES, SH>;
+  static_assert(C::fits, "tile shape does not fit this element type");
+  constexpr int kBlock = MfmaLaunch<T, D, P, SH>::KB;
   constexpr int NP = S::NP, NF = S::NF, NFP = S::NFP, NPERM = S::NPERM, NFF = M::NFF;
-  constexpr int NG = M::NG, NFK = M::NFK, KS = M::KS, MB = M::MB, RB = M::RB, NLO = M::NLO;
-  constexpr int NT4 = M::tail(sizeof(T) == 4, LO);  // 4-row tail pieces (0: all blocks 16x16x4)
-  constexpr int MBF = NT4 ? MB - 1 : MB;             // 16x16x4 row blocks
-  // A fragments per k-step: MBF 16x16x4 blocks of 64 lane values, and the tail piece of 16
-  // distinct values (row l%4 at lane group l/16; lanes 4 apart in l/4 read the same value)
-  constexpr int NAV = (MBF * 64 + NT4 * 16) * KS;    // operator values
-  constexpr int TE = MfmaLaunch<T, D, P>::TE;
+  constexpr int NG = M::NG, NFK = M::NFK, KS = M::KS, MB = M::MB, RB = C::RBP, NLO = M::NLO;
+  constexpr int NT4 = C::NT4, MBF = C::MBF, NAV = C::NAV, TE = C::TE;
   // row groups: RG waves share one group of 4 elements, wave rg computing row blocks
   // [rg*MBW, (rg+1)*MBW); with a single row block per wave the k-steps alternate between two
   // partial accumulators (summed before the write-out) so that consecutive MFMAs are independent
-  constexpr int RG = MfmaLaunch<T, D, P>::RG;
-  static_assert(RG == 1 || (NT4 == 0 && !LO && MBF % RG == 0), "row split: whole 16-row blocks only");
+  constexpr int RG = C::RG;
   constexpr int MBW = MBF / RG;
   constexpr int NCH = (RG > 1 && MBW == 1) ? 2 : 1;
   constexpr int VW = 16 / sizeof(T);                        // values per 16-byte vector
@@ -386,28 +442,17 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   constexpr int NGEO = (TE * 9 + kBlock - 1) / kBlock;      // geometry values per thread
   constexpr int NFC = (TE * NF + kBlock - 1) / kBlock;      // element faces per thread
   typedef typename Mfma<T>::V V;
-  // LDS element stride LES of the state tile and of the results: NP*ES values plus a pad (4-field
-  // layouts only, whose element boundaries fall on whole 16-byte vectors) chosen so that the four
-  // elements of a wave fall in different LDS banks. A volume k-step's lanes read node 4m+g of their
-  // element (16 or 32 bytes) and the result write-out puts row 4g+j (FP32) or g+4j (FP64) of 4
-  // elements x 4 fields: FP32 wants the stride in 4-byte words = 4 mod 16 (or 12), FP64 32 mod 64.
-  constexpr int PADV = lds_pad<T, ES>(NP);
-  constexpr int LES = NP * ES + PADV;
+  constexpr int LES = C::LES;
   __shared__ __attribute__((aligned(32))) T sU[TE * LES];
-  // face fluxes [element][field][RB], then the results in sU's layout. A face k-step's 64 lanes read
-  // words c*RB + g + const (c < 16, g < 4): for 4-byte values that is 64 distinct LDS banks whenever
-  // RB is 4 times an odd number (3D orders 4 and 5: 60, 84)
-  constexpr int NSB = 4 * TE * RB > TE * LES ? 4 * TE * RB : TE * LES;
+  // face fluxes [element][field][RB] (RB: the padded row stride rb_stride), then the results in
+  // sU's layout
+  constexpr int NSB = C::NSB;
   __shared__ __attribute__((aligned(16))) T sB[NSB];
   __shared__ T sN[TE * NF][4];
   __shared__ T sG[TE][9];
   __shared__ int sVmap[NF * NPERM * NFP];
   __shared__ int sFmask[NF * NFP];
-  // Operator fragments are staged in LDS whenever the whole workgroup footprint fits the 160 KiB
-  // of a gfx950 CU (fewer resident workgroups, but no per-k-step global loads).
-  constexpr size_t kLdsRest = sizeof(T) * (TE * LES + NSB + TE * NF * 4 + TE * 9) +
-                              sizeof(int) * (NF * NPERM * NFP + NF * NFP) + (LO ? 16 * MB * NLO * 64 : 0);
-  constexpr bool kALds = NAV * sizeof(T) + kLdsRest <= 160 * 1024 - 512;
+  constexpr bool kALds = C::kALds;
   __shared__ T sA[kALds ? NAV : 1];
   __shared__ bf16x8 sAlo[LO ? MB * NLO * 64 : 1];
   T* const sR = sB;
@@ -449,7 +494,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
   for (int t = 0; t < NE; ++t) {
     const int v = tid + t * kBlock;
     voff[t] = (uint32_t)(v * VW);
-    loff[t] = v * VW + (v * VW / (NP * ES)) * PADV;
+    loff[t] = v * VW + (v * VW / (NP * ES)) * C::PADV;
   }
 
   // Software pipeline: while tile t is processed, tile t+1's state, geometry and face geometry,
@@ -563,7 +608,7 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P>::KB)) void k_stage_mfma(StageA
     };
     // wide workgroups and operators read from global memory need the registers during the MFMA
     // chain: their epilogue operands are loaded later (below)
-    constexpr bool kDefer = !kALds || MfmaLaunch<T, D, P>::WAVES > 4;
+    constexpr bool kDefer = !kALds || MfmaLaunch<T, D, P, SH>::WAVES > 4;
     if (!kDefer) load_epi();
 #pragma unroll
     for (int t = 0; t < NI; ++t) cnc[t] = inc[t];
@@ -901,12 +946,13 @@ void launch_lds(StageArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((k_stage_lds<T, D, P>), dim3(8 * bpx), dim3(kBlock), 0, s, a);
 }
 
-template <typename T, int D, int P, bool LO, bool CS, int ES, bool BIG>
+template <typename T, int D, int P, bool LO, bool CS, int ES, bool BIG, int SH>
 void launch_mfma_kernel(StageArgs& a, hipStream_t s, int grid_pct) {
-  constexpr int TE = MfmaLaunch<T, D, P>::TE, KB = MfmaLaunch<T, D, P>::KB;
+  constexpr int TE = MfmaLaunch<T, D, P, SH>::TE, KB = MfmaLaunch<T, D, P, SH>::KB;
   static int occ = 0;
   if (!occ) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stage_mfma<T, D, P, LO, CS, ES, BIG>, KB, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stage_mfma<T, D, P, LO, CS, ES, BIG, SH>, KB, 0) !=
+            hipSuccess ||
         occ < 1)
       occ = 1;
   }
@@ -915,46 +961,101 @@ void launch_mfma_kernel(StageArgs& a, hipStream_t s, int grid_pct) {
   const int64_t per_xcd_cap = (int64_t)(num_cus() / 8) * occ * std::max(grid_pct, 1) / 100;
   int bpx = (int)std::min<int64_t>(a.tiles_per_xcd, std::max<int64_t>(per_xcd_cap, 1));
   if (bpx < 1) bpx = 1;
-  hipLaunchKernelGGL((k_stage_mfma<T, D, P, LO, CS, ES, BIG>), dim3(8 * bpx), dim3(KB), 0, s, a);
+  hipLaunchKernelGGL((k_stage_mfma<T, D, P, LO, CS, ES, BIG, SH>), dim3(8 * bpx), dim3(KB), 0, s, a);
+}
+
+// One of the other tile shapes (1-4). A shape that resolves to the built-in one's workgroup, tile
+// and operator placement runs the built-in kernel. Returns false, launching nothing, when the shape
+// does not fit this element type (LDS or whole row blocks).
+template <typename T, int D, int P, bool LO, bool CS, int ES, int SH>
+bool launch_shape(StageArgs& a, hipStream_t s, int grid_pct, bool dry) {
+  using L0 = MfmaLaunch<T, D, P, 0>;
+  using L = MfmaLaunch<T, D, P, SH>;
+  using C0 = MfmaCfg<T, D, P, LO, ES, 0>;
+  using C = MfmaCfg<T, D, P, LO, ES, SH>;
+  if constexpr (L::WAVES == L0::WAVES && L::TE == L0::TE && L::RG == L0::RG && C::kALds == C0::kALds) {
+    if (!dry) launch_mfma_kernel<T, D, P, LO, CS, ES, false, 0>(a, s, grid_pct);
+    return true;
+  } else if constexpr (C::fits) {
+    if (!dry) launch_mfma_kernel<T, D, P, LO, CS, ES, false, SH>(a, s, grid_pct);
+    return true;
+  } else {
+    return false;
+  }
 }
 
 // 32-bit offsets from the buffers' own bases while every array the kernel addresses (state,
-// J^-1, face geometry; the face tables are smaller) stays below 4 GiB, else per-tile 64-bit bases
+// J^-1, face geometry; the face tables are smaller) stays below 4 GiB, else per-tile 64-bit bases.
+// Tile shapes other than the built-in one are built for the forms the workloads run -- FP64, and
+// FP32 with the compensated solution, without the operator correction -- in 2D and 3D at orders
+// 2-5 with the initial live fields (2D: 3, 3D: 4), below 4 GiB per buffer. Returns false, launching
+// nothing, when `shape` has no kernel for this context's form.
 template <typename T, int D, int P, bool LO, bool CS, int ES>
-void launch_mfma_es(StageArgs& a, hipStream_t s, int grid_pct) {
-  const uint64_t lim = (uint64_t)1 << 32, kp = (uint64_t)a.Kpad;
-  const bool big = a.addr64 || kp * Sz<D, P>::NP * ES * sizeof(T) >= lim || kp * 9 * sizeof(T) >= lim ||
-                   kp * Sz<D, P>::NF * 4 * sizeof(T) >= lim;
-  if (!big) return launch_mfma_kernel<T, D, P, LO, CS, ES, false>(a, s, grid_pct);
+bool launch_mfma_es(StageArgs& a, hipStream_t s, int grid_pct, int shape, bool dry) {
+  const bool big = a.addr64 || stage_needs_addr64(a.Kpad, Sz<D, P>::NP, Sz<D, P>::NF, ES, (int)sizeof(T));
+  if (shape != 0) {
+    constexpr bool kShapes =
+        !LO && D >= 2 && P >= 2 && ES == (D == 3 ? 4 : D + 1) && (sizeof(T) == 8 ? !CS : CS);
+    if constexpr (kShapes) {
+      if (!big) switch (shape) {
+          case 1: return launch_shape<T, D, P, LO, CS, ES, 1>(a, s, grid_pct, dry);
+          case 2: return launch_shape<T, D, P, LO, CS, ES, 2>(a, s, grid_pct, dry);
+          case 3: return launch_shape<T, D, P, LO, CS, ES, 3>(a, s, grid_pct, dry);
+          case 4: return launch_shape<T, D, P, LO, CS, ES, 4>(a, s, grid_pct, dry);
+          default: break;
+        }
+    }
+    return false;
+  }
+  if (dry) return true;
+  if (!big) {
+    launch_mfma_kernel<T, D, P, LO, CS, ES, false, 0>(a, s, grid_pct);
+    return true;
+  }
   if constexpr (LO) throw Unsupported("the FP32 operator correction is limited to contexts below 4 GiB per buffer");
-  else launch_mfma_kernel<T, D, P, LO, CS, ES, true>(a, s, grid_pct);
+  else launch_mfma_kernel<T, D, P, LO, CS, ES, true, 0>(a, s, grid_pct);
+  return true;
 }
 
 // MFMA contexts keep the state node-interleaved with their live fields: ES = 4 in 3D, and in
 // 1D/2D dim+1 (the extra velocity components identically zero) or 4.
 template <typename T, int D, int P, bool LO, bool CS>
-void launch_mfma(StageArgs& a, hipStream_t s, int grid_pct, int es) {
+bool launch_mfma(StageArgs& a, hipStream_t s, int grid_pct, int es, int shape, bool dry) {
   if constexpr (D < 3)
-    if (es == D + 1) return launch_mfma_es<T, D, P, LO, CS, D + 1>(a, s, grid_pct);
+    if (es == D + 1) return launch_mfma_es<T, D, P, LO, CS, D + 1>(a, s, grid_pct, shape, dry);
   if (es != 4) throw InvalidArg("stage kernel: interleaved state with an unsupported field count");
-  launch_mfma_es<T, D, P, LO, CS, 4>(a, s, grid_pct);
+  return launch_mfma_es<T, D, P, LO, CS, 4>(a, s, grid_pct, shape, dry);
 }
 
 template <typename T, int D, int P>
-void launch_variant(Ctx& c, StageArgs& a, hipStream_t s) {
-  if (c.variant == kVariantLds) return launch_lds<T, D, P>(a, s);
+bool launch_mfma_form(Ctx& c, StageArgs& a, hipStream_t s, int shape, bool dry) {
   if constexpr (std::is_same<T, float>::value) {
-    if (c.comp && c.oplo) return launch_mfma<T, D, P, true, true>(a, s, c.grid_pct, c.es);
-    if (c.comp) return launch_mfma<T, D, P, false, true>(a, s, c.grid_pct, c.es);
-    if (c.oplo) return launch_mfma<T, D, P, true, false>(a, s, c.grid_pct, c.es);
+    if (c.comp && c.oplo) return launch_mfma<T, D, P, true, true>(a, s, c.grid_pct, c.es, shape, dry);
+    if (c.comp) return launch_mfma<T, D, P, false, true>(a, s, c.grid_pct, c.es, shape, dry);
+    if (c.oplo) return launch_mfma<T, D, P, true, false>(a, s, c.grid_pct, c.es, shape, dry);
   }
-  launch_mfma<T, D, P, false, false>(a, s, c.grid_pct, c.es);
+  return launch_mfma<T, D, P, false, false>(a, s, c.grid_pct, c.es, shape, dry);
+}
+
+// The context's tile shape, or the built-in one where the context's current form (live fields,
+// buffer sizes) has no kernel of that shape. With `dry`, launches nothing and tells whether
+// `shape` has a kernel for the context as it is.
+template <typename T, int D, int P>
+bool launch_variant(Ctx& c, StageArgs& a, hipStream_t s, int shape, bool dry) {
+  if (c.variant == kVariantLds) {
+    if (dry) return shape == 0;
+    launch_lds<T, D, P>(a, s);
+    return true;
+  }
+  if (launch_mfma_form<T, D, P>(c, a, s, shape, dry)) return true;
+  if (dry) return false;
+  return launch_mfma_form<T, D, P>(c, a, s, 0, false);
 }
 
 template <typename T>
-void dispatch(Ctx& c, StageArgs& a, hipStream_t s) {
+bool dispatch(Ctx& c, StageArgs& a, hipStream_t s, int shape, bool dry) {
 #define DGK_CASE(D, P) \
-  if (c.dim == D && c.p == P) return launch_variant<T, D, P>(c, a, s);
+  if (c.dim == D && c.p == P) return launch_variant<T, D, P>(c, a, s, shape, dry);
   DGK_CASE(1, 1) DGK_CASE(1, 2) DGK_CASE(1, 3) DGK_CASE(1, 4) DGK_CASE(1, 5)
   DGK_CASE(2, 1) DGK_CASE(2, 2) DGK_CASE(2, 3) DGK_CASE(2, 4) DGK_CASE(2, 5)
   DGK_CASE(3, 1) DGK_CASE(3, 2) DGK_CASE(3, 3) DGK_CASE(3, 4) DGK_CASE(3, 5)
@@ -965,9 +1066,15 @@ void dispatch(Ctx& c, StageArgs& a, hipStream_t s) {
 }  // namespace
 
 void launch_stage(Ctx& c, StageArgs& a, hipStream_t s) {
-  if (c.precision == DGK_FP64) dispatch<double>(c, a, s);
-  else dispatch<float>(c, a, s);
+  if (c.precision == DGK_FP64) dispatch<double>(c, a, s, c.tile_shape, false);
+  else dispatch<float>(c, a, s, c.tile_shape, false);
   DGK_HIP(hipGetLastError());
+}
+
+bool tile_shape_supported(Ctx& c, StageArgs& a, int shape) {
+  if (shape < 0 || shape >= kTileShapes) return false;
+  return c.precision == DGK_FP64 ? dispatch<double>(c, a, nullptr, shape, true)
+                                 : dispatch<float>(c, a, nullptr, shape, true);
 }
 
 void launch_source(Ctx& c, const Ctx::Src& src, double value, hipStream_t st) {

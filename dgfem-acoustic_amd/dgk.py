@@ -9,6 +9,7 @@ usable, the solver entry points raise; there is no CPU fallback.
 import ctypes
 import json
 import os
+import time
 
 import numpy as np
 
@@ -160,6 +161,9 @@ def gen_line(n, L, order, side_tags, names, config_text):
 FP64, FP32 = 0, 1
 SIGN_REFERENCE, SIGN_STABLE = 0, 1
 VARIANT_DEFAULT, VARIANT_LDS, VARIANT_MFMA = 0, 1, 2
+# MFMA stage-kernel tile shapes (dgk_set_tile_shape): built-in, 4 waves x 16 elements, 8 waves x 32
+# elements, row split (4 waves x 4 elements), 4 waves x 16 elements with the operator in global memory
+TILE_SHAPES = (0, 1, 2, 3, 4)
 # fp32_flags: 0 = default FP32 form (compensated solution); PLAIN_STATE = plain FP32 solution;
 # OPERATOR_CORRECTION = add the bf16 operator-residual MFMA
 FP32_PLAIN_STATE, FP32_OPERATOR_CORRECTION = 1, 2
@@ -194,7 +198,8 @@ DEV_SYMBOLS = ("dgk_create", "dgk_destroy", "dgk_last_error", "dgk_info_get", "d
                "dgk_batch_last_error", "dgk_batch_set_state", "dgk_batch_get_state", "dgk_batch_rk4",
                "dgk_batch_sync", "dgk_batch_groups", "dgk_batch_prepare", "dgk_batch_rk4_timed",
                "dgk_set_receivers", "dgk_get_receivers", "dgk_batch_set_receivers", "dgk_batch_get_receivers",
-               "dgk_set_launch", "dgk_load_tuning", "dgk_batch_set_launch")
+               "dgk_set_launch", "dgk_load_tuning", "dgk_batch_set_launch", "dgk_set_tile_shape",
+               "dgk_batch_set_tile_shape", "dgk_stage_addressing")
 
 
 def device():
@@ -238,6 +243,9 @@ def device():
         L.dgk_set_launch.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.dgk_load_tuning.argtypes = [ctypes.c_char_p]
         L.dgk_batch_set_launch.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.dgk_set_tile_shape.argtypes = [vp, ctypes.c_int]
+        L.dgk_batch_set_tile_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.dgk_stage_addressing.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64, P(ctypes.c_int)]
         if os.path.exists(TUNING_FILE) and not os.environ.get("DGK_NO_TUNING"):
             if L.dgk_load_tuning(TUNING_FILE.encode()):
                 raise RuntimeError(f"tuning table {TUNING_FILE}: {L.dgk_last_error(None).decode()}")
@@ -247,6 +255,17 @@ def device():
                                        P(ctypes.c_int), P(ctypes.c_int64), P(ctypes.c_int64)]
         _dev = L
     return _dev
+
+
+def stage_addressing(dim, order, precision, num_elements):
+    """True when a context of this size addresses the stage kernel's arrays through 64-bit
+    per-tile bases (some array reaches 4 GiB), False for 32-bit offsets (no device use)."""
+    L = device()
+    out = ctypes.c_int()
+    st = L.dgk_stage_addressing(int(dim), int(order), int(precision), int(num_elements), ctypes.byref(out))
+    if st:
+        raise DGKError(st, L.dgk_last_error(None).decode())
+    return bool(out.value)
 
 
 class DGKError(RuntimeError):
@@ -349,6 +368,11 @@ class Solver:
         """Launch shape of the stage kernel (tuning only; results are unchanged)."""
         self._check(device().dgk_set_launch(self.h, int(grid_pct), int(tile_order)))
 
+    def set_tile_shape(self, shape):
+        """Tile shape of the MFMA stage kernel (TILE_SHAPES; 0 the built-in one). Raises
+        DGKError (status DGK_ERR_UNSUPPORTED) when the context has no kernel of that shape."""
+        self._check(device().dgk_set_tile_shape(self.h, int(shape)))
+
     def rk4_timed(self, nsteps, dt):
         s, t = ctypes.c_double(), ctypes.c_double()
         self._check(device().dgk_rk4_timed(self.h, int(nsteps), float(dt), ctypes.byref(s), ctypes.byref(t)))
@@ -444,9 +468,29 @@ class Batch:
     def set_launch(self, group, grid_pct=100, tile_order=0):
         self._check(device().dgk_batch_set_launch(self.h, int(group), int(grid_pct), int(tile_order)))
 
+    def set_tile_shape(self, group, shape):
+        self._check(device().dgk_batch_set_tile_shape(self.h, int(group), int(shape)))
+
     def prepare(self, nsteps, schedule=SCHEDULE_PARALLEL):
         """Capture the graph for rk4(nsteps, schedule) ahead of time."""
         self._check(device().dgk_batch_prepare(self.h, int(nsteps), int(schedule)))
+
+    def pick_schedule(self, steps=10, margin=0.97):
+        """Schedule autotuning for this ensemble on this GPU: time `steps` captured steps with the
+        groups as parallel graph branches and `steps` in sequence (the members advance 2*steps
+        steps), and return (schedule, {schedule: seconds}). Parallel branches are taken only when
+        clearly faster (below `margin` of the sequence), since their per-kernel timings overlap."""
+        trial = {}
+        for sched in (SCHEDULE_PARALLEL, SCHEDULE_SEQUENTIAL):
+            self.prepare(steps, sched)
+            self.sync()
+            t0 = time.perf_counter()
+            self.rk4(steps, sched)
+            self.sync()
+            trial[sched] = time.perf_counter() - t0
+        best = SCHEDULE_PARALLEL if trial[SCHEDULE_PARALLEL] < margin * trial[SCHEDULE_SEQUENTIAL] else \
+            SCHEDULE_SEQUENTIAL
+        return best, trial
 
     def rk4_timed(self, nsteps):
         """Run nsteps per group with event-timed stage launches; summed stage ms per group."""

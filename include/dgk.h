@@ -129,8 +129,26 @@ dgk_status dgk_get_receivers(dgk_ctx* ctx, double* out, int capacity_rows, int* 
  * percentage of the occupancy-derived grid (100 = one full wave of resident workgroups) and the
  * tile order (0 = XCD-aware slabs, 1 = round-robin). Results do not depend on either. */
 dgk_status dgk_set_launch(dgk_ctx* ctx, int grid_pct, int tile_order);
-/* Load a tuning table (text lines "dim order precision variant grid_pct tile_order", '#'
- * comments), applied to contexts created afterwards: launch shape always, variant when
+/* Tile shape of the MFMA stage kernel (no reference counterpart; tuning): 0 the built-in choice
+ * per (precision, dim, order); 1 4-wave workgroups over 16-element tiles; 2 8-wave workgroups over
+ * 32-element tiles; 3 the row split (4 waves over a 4-element tile, each wave a quarter of the
+ * operator's 16-row blocks, the operator in LDS); 4 4-wave workgroups over 16-element tiles with
+ * the operator read from global memory. Shapes 1-4 exist for FP64 and the default FP32 form in 2D
+ * and 3D at orders 2-5, for buffers below 4 GiB; a shape that has no kernel for the context, or
+ * does not fit its element type, returns DGK_ERR_UNSUPPORTED and leaves the setting unchanged. If
+ * the context's form later changes to one without that shape (a 2D state with nonzero vz), the
+ * built-in shape runs. Results agree with the built-in shape to rounding (bitwise, except between
+ * the row split and the others, which sum the k-steps in a different order). */
+dgk_status dgk_set_tile_shape(dgk_ctx* ctx, int shape);
+/* Capacity planning (no reference counterpart; no device use): *addr64 = 1 when a context of
+ * num_elements elements of this dim/order/precision addresses its arrays through 64-bit per-tile
+ * bases (some array of the stage kernel -- state, J^-1 or face geometry -- reaches 4 GiB; the
+ * same arithmetic, bitwise equal results), 0 when 32-bit offsets suffice. Computed for the state's
+ * initial live fields (dim+1 in 1D/2D, 4 in 3D). DGK_ERR_UNSUPPORTED beyond 2^31 element faces. */
+dgk_status dgk_stage_addressing(int dim, int order, int precision, int64_t num_elements, int* addr64);
+/* Load a tuning table (text lines "dim order precision variant grid_pct tile_order [tile_shape
+ * stage_us | stage_us]", '#' comments), applied to contexts created afterwards: launch shape
+ * always, tile shape where the context has a kernel of that shape, variant when
  * dgk_opts.variant is 0. Replaces any table loaded before. */
 dgk_status dgk_load_tuning(const char* path);
 dgk_status dgk_connectivity(dgk_ctx* ctx, int32_t* nbr, int32_t* nface, int32_t* perm, int32_t* sign);
@@ -167,6 +185,8 @@ dgk_status dgk_batch_set_receivers(dgk_batch* b, int sim, int n, const double* x
 dgk_status dgk_batch_get_receivers(dgk_batch* b, int sim, double* out, int capacity_rows, int* rows);
 /* launch shape of one group's stage kernel (see dgk_set_launch) */
 dgk_status dgk_batch_set_launch(dgk_batch* b, int group, int grid_pct, int tile_order);
+/* tile shape of one group's stage kernel (see dgk_set_tile_shape) */
+dgk_status dgk_batch_set_tile_shape(dgk_batch* b, int group, int shape);
 /* capture (and keep) the graphs that dgk_batch_rk4(b, nsteps, schedule) will replay (up to 100 steps
  * each, the remainder in a graph of its own length), without running them */
 dgk_status dgk_batch_prepare(dgk_batch* b, int nsteps, int schedule);

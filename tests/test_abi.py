@@ -102,12 +102,48 @@ def test_shipped_tuning_table_covers_every_shape_and_loads(tmp_path):
     assert keys == sorted((d, p, q) for d in (1, 2, 3) for p in range(1, 6) for q in (dgk.FP64, dgk.FP32))
     for r in rows:
         assert int(r[3]) in (dgk.VARIANT_LDS, dgk.VARIANT_MFMA) and 1 <= int(r[4]) <= 10000 and int(r[5]) in (0, 1)
+        assert len(r) == 8 and int(r[6]) in dgk.TILE_SHAPES  # dim order precision variant grid order shape us
         if int(r[2]) == dgk.FP32:
             assert int(r[3]) == dgk.VARIANT_MFMA  # the compensated FP32 form runs on the MFMA kernel
     L = dgk.device()
     assert L.dgk_load_tuning(dgk.TUNING_FILE.encode()) == 0
-    for bad in ("3 3 1 9 100 0\n", "3 3 1 2 0 0\n", "3 3 1 2 100 2\n", "4 3 1 2 100 0\n", "3 3 1\n"):
+    for bad in ("3 3 1 9 100 0\n", "3 3 1 2 0 0\n", "3 3 1 2 100 2\n", "4 3 1 2 100 0\n", "3 3 1\n",
+                "3 3 1 2 100 0 5 210.5\n", "3 3 1 2 100 0 -1 210.5\n", "3 3 1 2 100 0 1.5 210.5\n",
+                "3 3 1 2 100 0 1 210.5 7\n"):
         f = tmp_path / "bad.txt"
         f.write_text(bad)
         assert L.dgk_load_tuning(str(f).encode()) == dgk.DGK_ERR_INVALID, bad
+    # rows with the tile-shape column, and rows of tables written before it (stage_us only)
+    for good in ("3 5 1 2 100 1 3 297.08\n", "3 5 1 2 100 1 297.08\n", "3 5 1 2 100 1\n"):
+        f = tmp_path / "good.txt"
+        f.write_text(good)
+        assert L.dgk_load_tuning(str(f).encode()) == 0, good
     assert L.dgk_load_tuning(dgk.TUNING_FILE.encode()) == 0
+
+
+def test_stage_addressing_switches_at_4_gib_per_array():
+    """The stage kernel's 64-bit per-tile addressing starts exactly where one of its arrays reaches
+    4 GiB: the state (Kpad*Np*fields*word) for 3D order 2, the J^-1 array (Kpad*9*word) for 1D
+    order 1, whose state is smaller; element counts are padded to multiples of 32. Beyond 2^31
+    element faces a context is refused (no device use)."""
+    ensure_device_lib()
+    lim = 1 << 32
+    # 3D P2 FP64: 10 nodes x 4 fields x 8 B = 320 B per element of state
+    k = lim // 320 // 32 * 32  # the largest padded count below the limit
+    assert not dgk.stage_addressing(3, 2, dgk.FP64, k)
+    assert dgk.stage_addressing(3, 2, dgk.FP64, k + 1)
+    assert not dgk.stage_addressing(3, 2, dgk.FP32, k + 1)  # half the bytes
+    # 1D P1 FP64: state 2 x 2 x 8 = 32 B per element, J^-1 9 x 8 = 72 B: J^-1 decides
+    k1 = lim // 72 // 32 * 32
+    assert 72 * k1 < lim <= 72 * (k1 + 32)
+    assert not dgk.stage_addressing(1, 1, dgk.FP64, k1)
+    assert dgk.stage_addressing(1, 1, dgk.FP64, k1 + 1)
+    # 2D P5 FP32: 21 nodes x 3 fields x 4 B = 252 B of state against 4 x 3 faces x 4 B of face geometry
+    k2 = -(-lim // 252)
+    assert dgk.stage_addressing(2, 5, dgk.FP32, k2) and not dgk.stage_addressing(2, 5, dgk.FP32, k2 - 32)
+    with pytest.raises(dgk.DGKError) as e:
+        dgk.stage_addressing(3, 1, dgk.FP64, (1 << 31) // 4)
+    assert e.value.status == dgk.DGK_ERR_UNSUPPORTED
+    with pytest.raises(dgk.DGKError) as e:
+        dgk.stage_addressing(4, 1, dgk.FP64, 10)
+    assert e.value.status == dgk.DGK_ERR_INVALID

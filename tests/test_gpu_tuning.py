@@ -1,5 +1,6 @@
 """Launch-shape tuning never changes results: every grid size and tile order gives bitwise the
-same state; a tuning table is applied at context creation."""
+same state; every tile shape agrees with the built-in one (bitwise, or to rounding where the row
+split sums in another order); a tuning table is applied at context creation."""
 import os
 
 import numpy as np
@@ -49,3 +50,109 @@ def test_tuning_table_is_loaded(tmp_path):
     s2.set_state(u)
     s2.rk4(10, 1e-6)
     assert np.array_equal(tuned, s2.get_state())
+
+
+SHAPE_CASES = [(d, p, prec) for d in (2, 3) for p in (2, 3, 4, 5) for prec in (dgk.FP64, dgk.FP32)]
+
+
+def _row_split(shape, dim, p, prec):
+    """Whether the MFMA kernel of this tile shape splits operator rows across waves (the built-in
+    shape does for FP64 3D order 5)."""
+    return shape == 3 or (shape == 0 and prec == dgk.FP64 and dim == 3 and p == 5)
+
+
+def _run(m, u, prec, shape=None, steps=30):
+    s = dgk.Solver(m, 340.0, 1.225, v0=(30.0, -10.0, 5.0), precision=prec, flux_sign=dgk.SIGN_STABLE)
+    try:
+        if shape is not None:
+            s.set_tile_shape(shape)
+        s.set_state(u)
+        s.rk4(steps, 1e-6)
+        return s.get_state()
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("dim,p,prec", SHAPE_CASES)
+def test_tile_shapes_agree_with_the_built_in_shape(dim, p, prec):
+    """Every tile shape with a kernel for the case computes the same discrete operator: bitwise the
+    built-in shape's state when both split elements across waves (the same k-step order per output),
+    within rounding when one of them is the row split (two partial sums per output): FP64 1e-12,
+    FP32 1e-5 relative L2 after 30 steps. A shape without a kernel is refused with
+    DGK_ERR_UNSUPPORTED and leaves the context on its shape."""
+    print(f"case dim={dim} p={p} prec={prec}", flush=True)
+    m = mixed_mesh(dim, p, 9)
+    u = np.random.default_rng(5).standard_normal((4, m.K * m.Np))
+    if dim == 2:
+        u[3] = 0  # the 2D form with its 3 live fields, which the other tile shapes are built for
+    ref = _run(m, u, prec)
+    ran = []
+    for sh in dgk.TILE_SHAPES[1:]:
+        try:
+            x = _run(m, u, prec, sh)
+        except dgk.DGKError as e:
+            assert e.status == dgk.DGK_ERR_UNSUPPORTED, (sh, e)
+            continue
+        ran.append(sh)
+        if _row_split(sh, dim, p, prec) == _row_split(0, dim, p, prec):
+            assert np.array_equal(x, ref), sh
+        else:
+            err = np.linalg.norm(x - ref) / np.linalg.norm(ref)
+            print(f"  shape {sh}: rel L2 {err:.2e} against the built-in shape", flush=True)
+            assert err <= (1e-12 if prec == dgk.FP64 else 1e-5), (sh, err)
+    # the 4-wave element-split shapes exist for every case here; the row split only where the
+    # operator is whole 16-row blocks (order 5 in 3D)
+    assert {1, 4} <= set(ran), ran
+    assert (3 in ran) == (dim == 3 and p == 5), ran
+
+
+def _batch(m2, u2, m3, u3, shapes=None):
+    b = dgk.Batch([m2, m3], [dict(c0=340.0, rho0=1.225, v0=(30.0, -10.0, 5.0))] * 2, [1e-6, 1e-6],
+                  precision=dgk.FP32, flux_sign=dgk.SIGN_STABLE)
+    try:
+        g = {(x["dim"], x["order"]): i for i, x in enumerate(b.groups())}
+        for key, sh in (shapes or {}).items():
+            b.set_tile_shape(g[key], sh)
+        b.set_state(0, u2)
+        b.set_state(1, u3)
+        b.rk4(30)
+        return b.get_state(0), b.get_state(1)
+    finally:
+        b.close()
+
+
+def test_tile_shape_fallback_and_batch_groups(tmp_path):
+    """A 2D context whose state gains a nonzero vz runs the built-in shape (bitwise the default); a
+    batch group takes a tile shape of its own (refused where it has no such kernel); the tuning
+    table's tile-shape column is applied at creation where the context has such a kernel and
+    ignored where it has none."""
+    m2 = mixed_mesh(2, 3, 9)
+    u2 = np.random.default_rng(6).standard_normal((4, m2.K * m2.Np))  # vz nonzero: 4 live fields
+    assert np.array_equal(_run(m2, u2, dgk.FP32, 2), _run(m2, u2, dgk.FP32))
+    m3 = mixed_mesh(3, 5, 9)
+    u3 = np.random.default_rng(7).standard_normal((4, m3.K * m3.Np))
+    u2[3] = 0
+    ref2, ref3 = _run(m2, u2, dgk.FP32), _run(m3, u3, dgk.FP32)
+    row3 = _run(m3, u3, dgk.FP32, 3)
+    assert not np.array_equal(row3, ref3)  # the row split sums in another order
+    for key, sh, status in (((2, 3), 3, dgk.DGK_ERR_UNSUPPORTED), ((3, 5), 9, dgk.DGK_ERR_INVALID)):
+        with pytest.raises(dgk.DGKError) as e:
+            _batch(m2, u2, m3, u3, {key: sh})
+        assert e.value.status == status
+    b2, b3 = _batch(m2, u2, m3, u3)
+    s2, s3 = _batch(m2, u2, m3, u3, {(2, 3): 2, (3, 5): 3})
+    assert np.array_equal(s2, b2)  # 8 waves x 32 elements: same k-step order
+    err = np.linalg.norm(s3 - b3) / np.linalg.norm(b3)
+    assert 0 < err <= 1e-5, err
+    L = dgk.device()
+    f = tmp_path / "t.txt"
+    f.write_text("3 5 1 2 100 1 3 1.0\n2 3 1 2 100 1 3 1.0\n")  # 2D order 3 has no row split
+    assert L.dgk_load_tuning(str(f).encode()) == 0
+    try:
+        t3, t2 = _run(m3, u3, dgk.FP32), _run(m2, u2, dgk.FP32)
+    finally:
+        empty = tmp_path / "empty.txt"
+        empty.write_text("")
+        assert L.dgk_load_tuning((dgk.TUNING_FILE if os.path.exists(dgk.TUNING_FILE) else str(empty)).encode()) == 0
+    assert np.array_equal(t3, row3)
+    assert np.array_equal(t2, ref2)
