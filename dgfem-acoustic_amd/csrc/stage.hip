@@ -325,19 +325,20 @@ __device__ __forceinline__ uint32_t f32_to_bf16_bits(float f) {
 // dim, order) below; 1 4-wave workgroups over 16-element tiles; 2 8-wave workgroups over
 // 32-element tiles; 3 the row split, 4 waves over a 4-element tile with wave w computing a quarter
 // of the operator's 16-row blocks (the whole operator in LDS); 4 4-wave workgroups over
-// 16-element tiles with the operator fragments read from global memory.
+// 16-element tiles with the operator fragments read from global memory; 5 the row split over
+// 8 waves and an 8-element tile (two element groups of 4, two waves per SIMD).
 // Built-in: 8-wave workgroups for 3D orders 4 and 5 (two waves per SIMD share one LDS copy of the
-// operator), the row split for FP64 3D order 5 (its 129 KB operator fits LDS only beside a small
-// tile), 4-wave workgroups elsewhere.
+// operator); for FP64 3D order 5 the 8-wave row split (shape 5: its operator, 110 KB with the
+// compact last row block, fits LDS only beside a small tile); 4-wave workgroups elsewhere.
 template <typename T, int D, int P, int SH = 0>
 struct MfmaLaunch {
   static constexpr bool DEF_ROWSPLIT = sizeof(T) == 8 && D == 3 && P == 5;
-  static constexpr int DEF_WAVES = (D == 3 && (P == 5 || P == 4) && !DEF_ROWSPLIT) ? 8 : 4;
-  static constexpr bool ROWSPLIT = SH == 3 || (SH == 0 && DEF_ROWSPLIT);
-  static constexpr int WAVES = SH == 0 ? DEF_WAVES : SH == 2 ? 8 : 4;
+  static constexpr int DEF_WAVES = (D == 3 && (P == 5 || P == 4)) ? 8 : 4;
+  static constexpr bool ROWSPLIT = SH == 3 || SH == 5 || (SH == 0 && DEF_ROWSPLIT);
+  static constexpr int WAVES = SH == 0 ? DEF_WAVES : (SH == 2 || SH == 5) ? 8 : 4;
   static constexpr bool OPG = SH == 4;                // operator fragments from global memory
   static constexpr int KB = 64 * WAVES;              // threads per workgroup
-  static constexpr int TE = ROWSPLIT ? 4 : 4 * WAVES;  // elements per tile
+  static constexpr int TE = ROWSPLIT ? WAVES : 4 * WAVES;  // elements per tile (row split: 4 row groups)
   static constexpr int RG = WAVES / (TE / 4);        // row groups: waves sharing one element group
 };
 
@@ -390,6 +391,13 @@ struct MfmaCfg {
   // distinct values (row l%4 at lane group l/16; lanes 4 apart in l/4 read the same value)
   static constexpr int NAV = (MBF * 64 + NT4 * 16) * M::KS;  // operator values
   static constexpr int TE = L::TE, RG = L::RG;
+  // Row split with one 16-row block per wave: when the last block holds at most 8 operator rows
+  // (3D order 5: rows 48-55), its fragments are kept in LDS without the zero rows, 32 values per
+  // k-step; lanes l of that wave read row l%8 (rows 56-63 of its result repeat 48-55 and are never
+  // written out). NAVS: the operator values held in LDS.
+  static constexpr int RT = S::NP - 16 * (M::MB - 1);
+  static constexpr bool kCompact = RG > 1 && RG == MBF && NT4 == 0 && RT <= 8;
+  static constexpr int NAVS = kCompact ? ((MBF - 1) * 64 + 32) * M::KS : NAV;
   // LDS element stride LES of the state tile and of the results: NP*ES values plus a pad (4-field
   // layouts only, whose element boundaries fall on whole 16-byte vectors) chosen so that the four
   // elements of a wave fall in different LDS banks. A volume k-step's lanes read node 4m+g of their
@@ -405,7 +413,7 @@ struct MfmaCfg {
   static constexpr size_t kLdsMax = 160 * 1024 - 512;
   // Operator fragments are staged in LDS whenever the whole workgroup footprint fits the 160 KiB
   // of a gfx950 CU (fewer resident workgroups, but no per-k-step global loads).
-  static constexpr bool kALds = !L::OPG && NAV * sizeof(T) + kLdsRest <= kLdsMax;
+  static constexpr bool kALds = !L::OPG && NAVS * sizeof(T) + kLdsRest <= kLdsMax;
   // the row split needs whole 16-row blocks per wave and the operator in LDS; the 8-wave tile shape
   // exists to share one LDS copy of the operator between two waves per SIMD (the built-in shape of
   // the FP32 3D order-5 operator correction runs 8 waves with the operator in global memory)
@@ -453,7 +461,8 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P, SH>::KB)) void k_stage_mfma(St
   __shared__ int sVmap[NF * NPERM * NFP];
   __shared__ int sFmask[NF * NFP];
   constexpr bool kALds = C::kALds;
-  __shared__ T sA[kALds ? NAV : 1];
+  constexpr bool kCompact = C::kCompact;
+  __shared__ T sA[kALds ? C::NAVS : 1];
   __shared__ bf16x8 sAlo[LO ? MB * NLO * 64 : 1];
   T* const sR = sB;
 
@@ -477,10 +486,18 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P, SH>::KB)) void k_stage_mfma(St
   constexpr int nq = ES;  // the buffers hold exactly the live fields
   for (int i = tid; i < NF * NPERM * NFP; i += kBlock) sVmap[i] = a.vmap[i];
   for (int i = tid; i < NF * NFP; i += kBlock) sFmask[i] = a.fmask[i];
-  if (kALds)
-    for (int i = tid; i < NAV; i += kBlock) sA[i] = Af[i];
+  if (kALds) {
+    constexpr int NFULL = kCompact ? (MBF - 1) * KS * 64 : NAV;
+    for (int i = tid; i < NFULL; i += kBlock) sA[i] = Af[i];
+    if constexpr (kCompact)  // the last block without its zero rows: value c of a k-step is row c%8, k c/8
+      for (int i = tid; i < KS * 32; i += kBlock)
+        sA[NFULL + i] = Af[NFULL + (i / 32) * 64 + ((i % 32) / 8) * 16 + (i % 8)];
+  }
   static_assert(RG == 1 || kALds, "row split keeps the operator in LDS");
-  const T* __restrict__ pA = (kALds ? (const T*)sA : Af) + lane + rg * (MBW * KS * 64);
+  // the wave of the compact last block (wave-uniform)
+  const bool clast = kCompact && rg == MBF - 1;
+  const T* __restrict__ pA = clast ? (const T*)sA + (MBF - 1) * KS * 64 + (lane >> 4) * 8 + (lane & 7)
+                                   : (kALds ? (const T*)sA : Af) + lane + rg * (MBW * KS * 64);
   const T* __restrict__ pT = (kALds ? (const T*)sA : Af) + MBF * KS * 64 + 4 * (lane >> 4) + (lane & 3);
   if (LO)
     for (int i = tid; i < MB * NLO * 64; i += kBlock) sAlo[i] = ((const bf16x8*)a.Alo)[i];
@@ -682,48 +699,60 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P, SH>::KB)) void k_stage_mfma(St
     const T* pAt = pA;
     const T* pTt = pT;
     if (!kALds) asm volatile("" : "+v"(pAt), "+v"(pTt));
-    auto step = [&](const int s, const T b) {
+    // the MFMA chain, with the A fragments AS values apart per k-step (64; 32 in the compact last
+    // block of the row split)
+    auto chain = [&](auto astride) {
+      constexpr int AS = decltype(astride)::value;
+      auto step = [&](const int s, const T b) {
 #pragma unroll
-      for (int mb = 0; mb < MBW; ++mb) {
-        const int ai = NCH > 1 ? mb * NCH + (s & 1) : mb;
-        acc[ai] = Mfma<T>::mma(pAt[(mb * KS + s) * 64], b, acc[ai]);
-      }
-#pragma unroll
-      for (int t = 0; t < NT4; ++t) acct[t] = Tail<T>::mma(pTt[(t * KS + s) * 16], b, acct[t]);
-      if constexpr (LO) {
-        blo[s % 8] = (__bf16)b;
-        if (s % 8 == 7 || s == KS - 1) {
-          if (s % 8 != 7)
-#pragma unroll
-            for (int i = s % 8 + 1; i < 8; ++i) blo[i] = (__bf16)0.0f;
-#pragma unroll
-          for (int mb = 0; mb < MB; ++mb) acc[mb] = mma_bf16(sAlo[(mb * NLO + s / 8) * 64 + lane], blo, acc[mb]);
+        for (int mb = 0; mb < MBW; ++mb) {
+          const int ai = NCH > 1 ? mb * NCH + (s & 1) : mb;
+          acc[ai] = Mfma<T>::mma(pAt[(mb * KS + s) * AS], b, acc[ai]);
         }
+#pragma unroll
+        for (int t = 0; t < NT4; ++t) acct[t] = Tail<T>::mma(pTt[(t * KS + s) * 16], b, acct[t]);
+        if constexpr (LO) {
+          blo[s % 8] = (__bf16)b;
+          if (s % 8 == 7 || s == KS - 1) {
+            if (s % 8 != 7)
+#pragma unroll
+              for (int i = s % 8 + 1; i < 8; ++i) blo[i] = (__bf16)0.0f;
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) acc[mb] = mma_bf16(sAlo[(mb * NLO + s / 8) * 64 + lane], blo, acc[mb]);
+          }
+        }
+      };
+      constexpr int kNodeUnroll = kALds ? NG : 2;
+#pragma unroll kNodeUnroll
+      for (int m = 0; m < NG; ++m) {
+        const int n = 4 * m + g;
+        T p = 0, vx = 0, vy = 0, vz = 0;
+        if (n < NP) {
+          T f[4];
+          node_fields<ES>(sU + el * LES + n * ES, f);
+          p = f[0];
+          vx = f[1];
+          vy = f[2];
+          vz = f[3];
+        }
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+          step(m * D + r, alpha[r] * p + beta[r][0] * vx + beta[r][1] * vy + beta[r][2] * vz);
+      }
+      // 8-wave workgroups: the epilogue operands are issued here, in flight during the face k-steps
+      // (the volume coefficients are dead by now, which keeps the peak within 256 registers)
+      if (kDefer && kALds) load_epi();
+#pragma unroll
+      for (int f = 0; f < NFK; ++f) {
+        const int jk = 4 * f + g;
+        step(D * NG + f, jk < NFF ? sB[(el * 4 + q) * RB + jk] : (T)0);
       }
     };
-    constexpr int kNodeUnroll = kALds ? NG : 2;
-#pragma unroll kNodeUnroll
-    for (int m = 0; m < NG; ++m) {
-      const int n = 4 * m + g;
-      T p = 0, vx = 0, vy = 0, vz = 0;
-      if (n < NP) {
-        T f[4];
-        node_fields<ES>(sU + el * LES + n * ES, f);
-        p = f[0];
-        vx = f[1];
-        vy = f[2];
-        vz = f[3];
-      }
-#pragma unroll
-      for (int r = 0; r < D; ++r) step(m * D + r, alpha[r] * p + beta[r][0] * vx + beta[r][1] * vy + beta[r][2] * vz);
-    }
-    // 8-wave workgroups: the epilogue operands are issued here, in flight during the face k-steps
-    // (the volume coefficients are dead by now, which keeps the peak within 256 registers)
-    if (kDefer && kALds) load_epi();
-#pragma unroll
-    for (int f = 0; f < NFK; ++f) {
-      const int jk = 4 * f + g;
-      step(D * NG + f, jk < NFF ? sB[(el * 4 + q) * RB + jk] : (T)0);
+    if constexpr (kCompact) {
+      if (clast) chain(std::integral_constant<int, 32>{});
+      else chain(std::integral_constant<int, 64>{});
+    } else {
+      chain(std::integral_constant<int, 64>{});
     }
     __syncthreads();  // face fluxes consumed; sB now receives the results in the LDS tile's layout
 #pragma unroll
@@ -1002,6 +1031,7 @@ bool launch_mfma_es(StageArgs& a, hipStream_t s, int grid_pct, int shape, bool d
           case 2: return launch_shape<T, D, P, LO, CS, ES, 2>(a, s, grid_pct, dry);
           case 3: return launch_shape<T, D, P, LO, CS, ES, 3>(a, s, grid_pct, dry);
           case 4: return launch_shape<T, D, P, LO, CS, ES, 4>(a, s, grid_pct, dry);
+          case 5: return launch_shape<T, D, P, LO, CS, ES, 5>(a, s, grid_pct, dry);
           default: break;
         }
     }

@@ -58,7 +58,7 @@ SHAPE_CASES = [(d, p, prec) for d in (2, 3) for p in (2, 3, 4, 5) for prec in (d
 def _row_split(shape, dim, p, prec):
     """Whether the MFMA kernel of this tile shape splits operator rows across waves (the built-in
     shape does for FP64 3D order 5)."""
-    return shape == 3 or (shape == 0 and prec == dgk.FP64 and dim == 3 and p == 5)
+    return shape in (3, 5) or (shape == 0 and prec == dgk.FP64 and dim == 3 and p == 5)
 
 
 def _run(m, u, prec, shape=None, steps=30):
@@ -100,10 +100,10 @@ def test_tile_shapes_agree_with_the_built_in_shape(dim, p, prec):
             err = np.linalg.norm(x - ref) / np.linalg.norm(ref)
             print(f"  shape {sh}: rel L2 {err:.2e} against the built-in shape", flush=True)
             assert err <= (1e-12 if prec == dgk.FP64 else 1e-5), (sh, err)
-    # the 4-wave element-split shapes exist for every case here; the row split only where the
-    # operator is whole 16-row blocks (order 5 in 3D)
+    # the 4-wave element-split shapes exist for every case here; the row splits (4 and 8 waves) only
+    # where the operator is whole 16-row blocks (order 5 in 3D)
     assert {1, 4} <= set(ran), ran
-    assert (3 in ran) == (dim == 3 and p == 5), ran
+    assert (3 in ran) == (5 in ran) == (dim == 3 and p == 5), ran
 
 
 def _batch(m2, u2, m3, u3, shapes=None):
