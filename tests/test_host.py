@@ -167,6 +167,41 @@ def test_first_node_bc_rule_matches_oracle():
         assert {a for a, _ in got} == {0, 1}
 
 
+def test_first_node_bc_rule_by_hand():
+    """Boundary codes of a 2 x 1 rectangle derived from coordinates alone: the sides y = 0 and
+    y = 1 are in group 1 (Wall, reflecting), x = 0 and x = 1 in group 2 (Abs, absorbing). A
+    boundary face takes the code of its first node; a corner node lies in both groups, and the
+    group applied last (ascending physical tag: Abs) wins (src/Mesh.cpp:364-384). So a face on a
+    Wall side is reflecting exactly when its first node is not a corner, and every face on an Abs
+    side is absorbing. Checked against the oracle's face table and the host's per-node codes."""
+    for order in (1, 2, 3):
+        m = dgk.gen_rect(2, 1, 1.0, 1.0, order, [1, 2, 1, 2], "1:Wall;2:Abs",
+                         conf("Wall=Reflecting\nAbs=Absorbing\n"))
+        c = ob.Oracle(m, 340.0, 1.225).connectivity()
+        r = dgk.ref_element(2, order)
+        xyz = m.node_xyz.reshape(m.K, m.Np, 3)
+        on = lambda v, a: abs(v - a) < 1e-12  # noqa: E731
+        seen = {0: 0, 1: 0}
+        wall_corner = 0
+        for e in range(m.K):
+            for j in range(3):
+                a, b = xyz[e, r["fmask"][j, 0]], xyz[e, r["fmask"][j, -1]]
+                wall_side = (on(a[1], 0) and on(b[1], 0)) or (on(a[1], 1) and on(b[1], 1))
+                abs_side = (on(a[0], 0) and on(b[0], 0)) or (on(a[0], 1) and on(b[0], 1))
+                f = c["elFIds"][e, j]
+                if not (wall_side or abs_side):
+                    assert c["fBC"][f] < 0  # interior face
+                    continue
+                corner = (on(a[0], 0) or on(a[0], 1)) and (on(a[1], 0) or on(a[1], 1))
+                want = 1 if wall_side and not corner else 0
+                wall_corner += wall_side and corner
+                assert c["fBC"][f] == want, (order, e, j, a, b)
+                assert m.node_bc[e * m.Np + r["fmask"][j, 0]] == want
+                seen[want] += 1
+        assert seen[0] > 0 and seen[1] > 0  # both outcomes occur on this mesh
+        assert wall_corner > 0  # and the corner rule decides some Wall-side face
+
+
 # ----------------------------------------------------------------------------- CLI
 def test_cli_wrong_argc_returns_e2big():
     exe = os.path.join(dgk.PKG_DIR, "dgalerkin")
