@@ -326,16 +326,21 @@ __device__ __forceinline__ uint32_t f32_to_bf16_bits(float f) {
 // 32-element tiles; 3 the row split, 4 waves over a 4-element tile with wave w computing a quarter
 // of the operator's 16-row blocks (the whole operator in LDS); 4 4-wave workgroups over
 // 16-element tiles with the operator fragments read from global memory; 5 the row split over
-// 8 waves and an 8-element tile (two element groups of 4, two waves per SIMD).
-// Built-in: 8-wave workgroups for 3D orders 4 and 5 (two waves per SIMD share one LDS copy of the
-// operator); for FP64 3D order 5 the 8-wave row split (shape 5: its operator, 110 KB with the
-// compact last row block, fits LDS only beside a small tile); 4-wave workgroups elsewhere.
+// 8 waves and an 8-element tile (two element groups of 4, two waves per SIMD); 6 4-wave workgroups
+// over 16-element tiles compiled for three waves per SIMD (168 registers), so that three
+// workgroups share a CU where LDS allows.
+// Built-in: 8-wave workgroups for 3D order 5 and FP64 3D order 4 (two waves per SIMD share one LDS
+// copy of the operator); for FP64 3D order 5 the 8-wave row split (shape 5: its operator, 110 KB
+// with the compact last row block, fits LDS only beside a small tile); shape 6 for FP32 3D order 4
+// (three 52 KB workgroups per CU); 4-wave workgroups elsewhere.
 template <typename T, int D, int P, int SH = 0>
 struct MfmaLaunch {
   static constexpr bool DEF_ROWSPLIT = sizeof(T) == 8 && D == 3 && P == 5;
-  static constexpr int DEF_WAVES = (D == 3 && (P == 5 || P == 4)) ? 8 : 4;
+  static constexpr int DEF_WAVES = (D == 3 && (P == 5 || (P == 4 && sizeof(T) == 8))) ? 8 : 4;
+  static constexpr int DEF_MINW = (sizeof(T) == 4 && D == 3 && P == 4) ? 3 : 1;
   static constexpr bool ROWSPLIT = SH == 3 || SH == 5 || (SH == 0 && DEF_ROWSPLIT);
   static constexpr int WAVES = SH == 0 ? DEF_WAVES : (SH == 2 || SH == 5) ? 8 : 4;
+  static constexpr int MINW = SH == 0 ? DEF_MINW : SH == 6 ? 3 : 1;  // waves per SIMD the registers are sized for
   static constexpr bool OPG = SH == 4;                // operator fragments from global memory
   static constexpr int KB = 64 * WAVES;              // threads per workgroup
   static constexpr int TE = ROWSPLIT ? WAVES : 4 * WAVES;  // elements per tile (row split: 4 row groups)
@@ -417,12 +422,16 @@ struct MfmaCfg {
   // the row split needs whole 16-row blocks per wave and the operator in LDS; the 8-wave tile shape
   // exists to share one LDS copy of the operator between two waves per SIMD (the built-in shape of
   // the FP32 3D order-5 operator correction runs 8 waves with the operator in global memory)
+  // a register budget for MINW waves per SIMD needs that many workgroups' LDS on one CU
   static constexpr bool fits = kLdsRest <= kLdsMax && (RG == 1 || (NT4 == 0 && !LO && MBF % RG == 0 && kALds)) &&
-                               (SH == 0 || L::WAVES == 4 || kALds);
+                               (SH == 0 || L::WAVES == 4 || kALds) &&
+                               (SH == 0 || L::MINW * 4 / L::WAVES * ((kALds ? NAVS * sizeof(T) : 0) + kLdsRest) <=
+                                               160 * 1024);
 };
 
 template <typename T, int D, int P, bool LO, bool CS, int ES, bool BIG, int SH>
-__global__ __launch_bounds__((MfmaLaunch<T, D, P, SH>::KB)) void k_stage_mfma(StageArgs a) {
+__global__ __launch_bounds__((MfmaLaunch<T, D, P, SH>::KB), (MfmaLaunch<T, D, P, SH>::MINW)) void k_stage_mfma(
+    StageArgs a) {
   static_assert(!CS || sizeof(T) == 4, "the compensated state is an FP32 form");
   using S = Sz<D, P>;
   using M = MfmaShape<D, P>;
@@ -1002,7 +1011,8 @@ bool launch_shape(StageArgs& a, hipStream_t s, int grid_pct, bool dry) {
   using L = MfmaLaunch<T, D, P, SH>;
   using C0 = MfmaCfg<T, D, P, LO, ES, 0>;
   using C = MfmaCfg<T, D, P, LO, ES, SH>;
-  if constexpr (L::WAVES == L0::WAVES && L::TE == L0::TE && L::RG == L0::RG && C::kALds == C0::kALds) {
+  if constexpr (L::WAVES == L0::WAVES && L::TE == L0::TE && L::RG == L0::RG && L::MINW == L0::MINW &&
+                C::kALds == C0::kALds) {
     if (!dry) launch_mfma_kernel<T, D, P, LO, CS, ES, false, 0>(a, s, grid_pct);
     return true;
   } else if constexpr (C::fits) {
@@ -1032,6 +1042,7 @@ bool launch_mfma_es(StageArgs& a, hipStream_t s, int grid_pct, int shape, bool d
           case 3: return launch_shape<T, D, P, LO, CS, ES, 3>(a, s, grid_pct, dry);
           case 4: return launch_shape<T, D, P, LO, CS, ES, 4>(a, s, grid_pct, dry);
           case 5: return launch_shape<T, D, P, LO, CS, ES, 5>(a, s, grid_pct, dry);
+          case 6: return launch_shape<T, D, P, LO, CS, ES, 6>(a, s, grid_pct, dry);
           default: break;
         }
     }
