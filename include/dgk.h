@@ -134,13 +134,15 @@ dgk_status dgk_set_launch(dgk_ctx* ctx, int grid_pct, int tile_order);
  * 32-element tiles; 3 the row split (4 waves over a 4-element tile, each wave a quarter of the
  * operator's 16-row blocks, the operator in LDS); 4 4-wave workgroups over 16-element tiles with
  * the operator read from global memory; 5 the row split over 8-wave workgroups and 8-element tiles
- * (two element groups, two waves per SIMD); 6 4-wave workgroups over 16-element tiles compiled for
- * three waves per SIMD (168 registers; where three workgroups' LDS fits a CU). Shapes 1-6 exist for FP64 and the default FP32 form in
+ * (two element groups, two waves per SIMD); 6, 7, 8 4-wave workgroups over 16-element tiles compiled
+ * for 3, 4 or 5 waves per SIMD (at most 512/n registers per lane; where that many workgroups' LDS
+ * fits a CU). Shapes 1-8 exist for FP64 and the default FP32 form in
  * 2D and 3D at orders 2-5, for buffers below 4 GiB; a shape that has no kernel for the context, or
  * does not fit its element type, returns DGK_ERR_UNSUPPORTED and leaves the setting unchanged. If
  * the context's form later changes to one without that shape (a 2D state with nonzero vz), the
- * built-in shape runs. Results agree with the built-in shape to rounding (bitwise, except between
- * the row split and the others, which sum the k-steps in a different order). */
+ * built-in shape runs. Results agree with the built-in shape to rounding: bitwise between
+ * element-split shapes compiled for the same waves per SIMD; the row split sums the k-steps in
+ * another order, and another register budget may fuse a product into a multiply-add differently. */
 dgk_status dgk_set_tile_shape(dgk_ctx* ctx, int shape);
 /* Capacity planning (no reference counterpart; no device use): *addr64 = 1 when a context of
  * num_elements elements of this dim/order/precision addresses its arrays through 64-bit per-tile

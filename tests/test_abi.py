@@ -108,7 +108,7 @@ def test_shipped_tuning_table_covers_every_shape_and_loads(tmp_path):
     L = dgk.device()
     assert L.dgk_load_tuning(dgk.TUNING_FILE.encode()) == 0
     for bad in ("3 3 1 9 100 0\n", "3 3 1 2 0 0\n", "3 3 1 2 100 2\n", "4 3 1 2 100 0\n", "3 3 1\n",
-                "3 3 1 2 100 0 6 210.5\n", "3 3 1 2 100 0 -1 210.5\n", "3 3 1 2 100 0 1.5 210.5\n",
+                f"3 3 1 2 100 0 {len(dgk.TILE_SHAPES)} 210.5\n", "3 3 1 2 100 0 -1 210.5\n", "3 3 1 2 100 0 1.5 210.5\n",
                 "3 3 1 2 100 0 1 210.5 7\n"):
         f = tmp_path / "bad.txt"
         f.write_text(bad)

@@ -61,6 +61,14 @@ def _row_split(shape, dim, p, prec):
     return shape in (3, 5) or (shape == 0 and prec == dgk.FP64 and dim == 3 and p == 5)
 
 
+def _waves_per_simd_budget(shape, dim, p, prec):
+    """Waves per SIMD the tile shape's registers are sized for (shapes 6-8: 3-5; the built-in shape
+    of FP32 3D order 4 is shape 6)."""
+    if shape == 0:
+        return 3 if (prec == dgk.FP32 and dim == 3 and p == 4) else 1
+    return shape - 3 if shape in (6, 7, 8) else 1
+
+
 def _run(m, u, prec, shape=None, steps=30):
     s = dgk.Solver(m, 340.0, 1.225, v0=(30.0, -10.0, 5.0), precision=prec, flux_sign=dgk.SIGN_STABLE)
     try:
@@ -76,10 +84,12 @@ def _run(m, u, prec, shape=None, steps=30):
 @pytest.mark.parametrize("dim,p,prec", SHAPE_CASES)
 def test_tile_shapes_agree_with_the_built_in_shape(dim, p, prec):
     """Every tile shape with a kernel for the case computes the same discrete operator: bitwise the
-    built-in shape's state when both split elements across waves (the same k-step order per output),
-    within rounding when one of them is the row split (two partial sums per output): FP64 1e-12,
-    FP32 1e-5 relative L2 after 30 steps. A shape without a kernel is refused with
-    DGK_ERR_UNSUPPORTED and leaves the context on its shape."""
+    built-in shape's state when both split elements across waves (the same k-step order per output)
+    and are compiled for the same waves per SIMD; otherwise within rounding (FP64 1e-12, FP32 1e-5
+    relative L2 after 30 steps): the row split sums the k-steps in two partial accumulators, and
+    another register budget may fuse a product into a multiply-add differently (measured: shape 8
+    differs from the built-in shape by 1e-16 FP64 and 1e-8 FP32 relative after 30 steps). A shape
+    without a kernel is refused with DGK_ERR_UNSUPPORTED and leaves the context on its shape."""
     print(f"case dim={dim} p={p} prec={prec}", flush=True)
     m = mixed_mesh(dim, p, 9)
     u = np.random.default_rng(5).standard_normal((4, m.K * m.Np))
@@ -94,7 +104,8 @@ def test_tile_shapes_agree_with_the_built_in_shape(dim, p, prec):
             assert e.status == dgk.DGK_ERR_UNSUPPORTED, (sh, e)
             continue
         ran.append(sh)
-        if _row_split(sh, dim, p, prec) == _row_split(0, dim, p, prec):
+        if (_row_split(sh, dim, p, prec) == _row_split(0, dim, p, prec) and
+                _waves_per_simd_budget(sh, dim, p, prec) == _waves_per_simd_budget(0, dim, p, prec)):
             assert np.array_equal(x, ref), sh
         else:
             err = np.linalg.norm(x - ref) / np.linalg.norm(ref)
