@@ -48,7 +48,7 @@ inline bool stage_needs_addr64(int64_t Kpad, int np, int nf, int es, int tsize) 
 // 2 8 waves x 32 elements, 3 row split (4 waves x 4 elements), 4 4 waves x 16 elements with the
 // operator read from global memory, 5 row split over 8 waves x 8 elements, 6-8 4 waves x 16
 // elements compiled for 3-5 waves per SIMD
-constexpr int kTileShapes = 11;
+constexpr int kTileShapes = 9;
 constexpr int kMaxNFP = 21;
 constexpr int kMaxPerm = 6;
 

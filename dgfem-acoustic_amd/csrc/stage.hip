@@ -339,7 +339,7 @@ struct MfmaLaunch {
   static constexpr int DEF_WAVES = (D == 3 && (P == 5 || (P == 4 && sizeof(T) == 8))) ? 8 : 4;
   static constexpr int DEF_MINW = (sizeof(T) == 4 && D == 3 && P == 4) ? 3 : 1;
   static constexpr bool ROWSPLIT = SH == 3 || SH == 5 || (SH == 0 && DEF_ROWSPLIT);
-  static constexpr int WAVES = SH == 0 ? DEF_WAVES : (SH == 2 || SH == 5) ? 8 : SH == 9 ? 1 : SH == 10 ? 2 : 4;
+  static constexpr int WAVES = SH == 0 ? DEF_WAVES : (SH == 2 || SH == 5) ? 8 : 4;
   static constexpr int MINW = SH == 0 ? DEF_MINW : (SH >= 6 && SH <= 8) ? SH - 3 : 1;  // waves per SIMD the registers are sized for
   static constexpr bool OPG = SH == 4;                // operator fragments from global memory
   static constexpr int KB = 64 * WAVES;              // threads per workgroup
@@ -1045,8 +1045,6 @@ bool launch_mfma_es(StageArgs& a, hipStream_t s, int grid_pct, int shape, bool d
           case 6: return launch_shape<T, D, P, LO, CS, ES, 6>(a, s, grid_pct, dry);
           case 7: return launch_shape<T, D, P, LO, CS, ES, 7>(a, s, grid_pct, dry);
           case 8: return launch_shape<T, D, P, LO, CS, ES, 8>(a, s, grid_pct, dry);
-          case 9: return launch_shape<T, D, P, LO, CS, ES, 9>(a, s, grid_pct, dry);
-          case 10: return launch_shape<T, D, P, LO, CS, ES, 10>(a, s, grid_pct, dry);
           default: break;
         }
     }

@@ -69,7 +69,9 @@ def _waves_per_simd_budget(shape, dim, p, prec):
     return shape - 3 if shape in (6, 7, 8) else 1
 
 
-def _run(m, u, prec, shape=None, steps=30):
+def _run(m, u, prec, shape=0, steps=30):
+    """RK4 steps on tile shape `shape` (0: the built-in one, whatever the tuning table says; None:
+    the shape the context was created with)."""
     s = dgk.Solver(m, 340.0, 1.225, v0=(30.0, -10.0, 5.0), precision=prec, flux_sign=dgk.SIGN_STABLE)
     try:
         if shape is not None:
@@ -146,11 +148,11 @@ def test_tile_shape_fallback_and_batch_groups(tmp_path):
     ref2, ref3 = _run(m2, u2, dgk.FP32), _run(m3, u3, dgk.FP32)
     row3 = _run(m3, u3, dgk.FP32, 3)
     assert not np.array_equal(row3, ref3)  # the row split sums in another order
-    for key, sh, status in (((2, 3), 3, dgk.DGK_ERR_UNSUPPORTED), ((3, 5), 9, dgk.DGK_ERR_INVALID)):
+    for key, sh, status in (((2, 3), 3, dgk.DGK_ERR_UNSUPPORTED), ((3, 5), len(dgk.TILE_SHAPES), dgk.DGK_ERR_INVALID)):
         with pytest.raises(dgk.DGKError) as e:
             _batch(m2, u2, m3, u3, {key: sh})
         assert e.value.status == status
-    b2, b3 = _batch(m2, u2, m3, u3)
+    b2, b3 = _batch(m2, u2, m3, u3, {(2, 3): 0, (3, 5): 0})  # the built-in shapes
     s2, s3 = _batch(m2, u2, m3, u3, {(2, 3): 2, (3, 5): 3})
     assert np.array_equal(s2, b2)  # 8 waves x 32 elements: same k-step order
     err = np.linalg.norm(s3 - b3) / np.linalg.norm(b3)
@@ -160,7 +162,7 @@ def test_tile_shape_fallback_and_batch_groups(tmp_path):
     f.write_text("3 5 1 2 100 1 3 1.0\n2 3 1 2 100 1 3 1.0\n")  # 2D order 3 has no row split
     assert L.dgk_load_tuning(str(f).encode()) == 0
     try:
-        t3, t2 = _run(m3, u3, dgk.FP32), _run(m2, u2, dgk.FP32)
+        t3, t2 = _run(m3, u3, dgk.FP32, None), _run(m2, u2, dgk.FP32, None)
     finally:
         empty = tmp_path / "empty.txt"
         empty.write_text("")
