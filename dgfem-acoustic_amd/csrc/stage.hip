@@ -668,15 +668,18 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P, SH>::KB), (MfmaLaunch<T, D, P,
       T qpv[3];
       for (int c = 0; c < 3; ++c) qpv[c] = inter ? iqp[t][1 + c] : qm[1 + c] - gm * unm * n[c];
       const T unp = qpv[0] * n[0] + qpv[1] * n[1] + qpv[2] * n[2];
-      const T A = (inter || refl) ? (T)0.5 : (T)0.25;
-      const T H = inter ? (T)0.5 * ((nc.y >> 16) & 1 ? -c0 : c0) : (refl ? (T)0 : (T)0.25 * c0);
-      const T vne = (inter || refl) ? vn : (T)0;
+      // the face scale fs is folded into A and H; sums before products:
+      //   fs fl_p = fs A (vne (pm + pp) + rc2 (un_m + un_p)) + fs H (pm - pp)
+      //   fs fl_v = fs A (vne (vm + vp) + (pm + pp)/rho n) + fs H (vm - vp)
+      const T A = fs * ((inter || refl) ? (T)0.5 : (T)0.25);
+      const T H = fs * (inter ? (T)0.5 * ((nc.y >> 16) & 1 ? -c0 : c0) : (refl ? (T)0 : (T)0.25 * c0));
+      const T Av = (inter || refl) ? A * vn : (T)0;
+      const T ps = qm[0] + qp0, pd = qm[0] - qp0;
+      const T Ap = A * irho * ps;
       T fl[4];
-      fl[0] = A * ((vne * qm[0] + rc2 * unm) + (vne * qp0 + rc2 * unp)) + H * (qm[0] - qp0);
-      for (int c = 0; c < 3; ++c)
-        fl[1 + c] = A * ((vne * qm[1 + c] + qm[0] * irho * n[c]) + (vne * qpv[c] + qp0 * irho * n[c])) +
-                    H * (qm[1 + c] - qpv[c]);
-      for (int qq = 0; qq < 4; ++qq) sB[(e * 4 + qq) * RB + jk] = fs * fl[qq];
+      fl[0] = Av * ps + (A * rc2) * (unm + unp) + H * pd;
+      for (int c = 0; c < 3; ++c) fl[1 + c] = Av * (qm[1 + c] + qpv[c]) + Ap * n[c] + H * (qm[1 + c] - qpv[c]);
+      for (int qq = 0; qq < 4; ++qq) sB[(e * 4 + qq) * RB + jk] = fl[qq];
     }
     if (more) gather();  // next tile's neighbour traces, in flight during the MFMA phase
     __syncthreads();
