@@ -10,8 +10,8 @@ when it beats the built-in shape's best by 2%):
   tile_shape  the MFMA kernel's workgroup and tile (dgk_set_tile_shape): 0 built-in, 1 4 waves x 16
               elements, 2 8 waves x 32 elements, 3 row split (4 waves x 4 elements), 4 4 waves x 16
               elements with the operator in global memory, 5 row split over 8 waves x 8 elements,
-              6, 7, 8 4 waves x 16 elements compiled for 3, 4 or 5 waves per SIMD, 9, 10 as 7, 8
-              with the RK epilogue operands loaded late; shapes without a kernel for the case are
+              6, 7, 8 4 waves x 16 elements compiled for 3, 4 or 5 waves per SIMD, 9, 10, 11 as
+              7, 8, 6 with the RK epilogue operands loaded late; shapes without a kernel for the case are
               skipped
   grid_pct    persistent grid as a percentage of the occupancy-derived grid: 50, 100, 200, 400
   tile_order  XCD-aware slabs (0) or round-robin (1)

@@ -47,8 +47,9 @@ inline bool stage_needs_addr64(int64_t Kpad, int np, int nf, int es, int tsize) 
 // MFMA stage-kernel tile shapes (dgk_set_tile_shape): 0 built-in, 1 4 waves x 16 elements,
 // 2 8 waves x 32 elements, 3 row split (4 waves x 4 elements), 4 4 waves x 16 elements with the
 // operator read from global memory, 5 row split over 8 waves x 8 elements, 6-8 4 waves x 16
-// elements compiled for 3-5 waves per SIMD, 9-10 as 7-8 with the epilogue operands loaded late
-constexpr int kTileShapes = 11;
+// elements compiled for 3-5 waves per SIMD, 9, 10, 11 as 7, 8, 6 with the epilogue operands
+// loaded late
+constexpr int kTileShapes = 12;
 constexpr int kMaxNFP = 21;
 constexpr int kMaxPerm = 6;
 

@@ -328,8 +328,8 @@ __device__ __forceinline__ uint32_t f32_to_bf16_bits(float f) {
 // 16-element tiles with the operator fragments read from global memory; 5 the row split over
 // 8 waves and an 8-element tile (two element groups of 4, two waves per SIMD); 6, 7, 8 4-wave
 // workgroups over 16-element tiles compiled for 3, 4 or 5 waves per SIMD (at most 512/n registers
-// per lane), so that as many workgroups share a CU where LDS allows; 9, 10 as 7, 8 with the RK
-// epilogue operands loaded after the volume k-steps (fewer registers live across the chain).
+// per lane), so that as many workgroups share a CU where LDS allows; 9, 10, 11 as 7, 8, 6 with the
+// RK epilogue operands loaded after the volume k-steps (fewer registers live across the chain).
 // Built-in: 8-wave workgroups for 3D order 5 and FP64 3D order 4 (two waves per SIMD share one LDS
 // copy of the operator); for FP64 3D order 5 the 8-wave row split (shape 5: its operator, 110 KB
 // with the compact last row block, fits LDS only beside a small tile); shape 6 for FP32 3D order 4
@@ -341,8 +341,8 @@ struct MfmaLaunch {
   static constexpr int DEF_MINW = (sizeof(T) == 4 && D == 3 && P == 4) ? 3 : 1;
   static constexpr bool ROWSPLIT = SH == 3 || SH == 5 || (SH == 0 && DEF_ROWSPLIT);
   static constexpr int WAVES = SH == 0 ? DEF_WAVES : (SH == 2 || SH == 5) ? 8 : 4;
-  static constexpr int MINW = SH == 0 ? DEF_MINW : (SH >= 6 && SH <= 8) ? SH - 3 : (SH == 9 || SH == 10) ? SH - 5 : 1;
-  static constexpr bool DEFER = SH == 9 || SH == 10;  // epilogue operands loaded after the volume k-steps  // waves per SIMD the registers are sized for
+  static constexpr int MINW = SH == 0 ? DEF_MINW : (SH >= 6 && SH <= 8) ? SH - 3 : (SH == 9 || SH == 10) ? SH - 5 : SH == 11 ? 3 : 1;
+  static constexpr bool DEFER = SH == 9 || SH == 10 || SH == 11;  // epilogue operands loaded after the volume k-steps  // waves per SIMD the registers are sized for
   static constexpr bool OPG = SH == 4;                // operator fragments from global memory
   static constexpr int KB = 64 * WAVES;              // threads per workgroup
   static constexpr int TE = ROWSPLIT ? WAVES : 4 * WAVES;  // elements per tile (row split: 4 row groups)
@@ -1052,6 +1052,7 @@ bool launch_mfma_es(StageArgs& a, hipStream_t s, int grid_pct, int shape, bool d
           case 8: return launch_shape<T, D, P, LO, CS, ES, 8>(a, s, grid_pct, dry);
           case 9: return launch_shape<T, D, P, LO, CS, ES, 9>(a, s, grid_pct, dry);
           case 10: return launch_shape<T, D, P, LO, CS, ES, 10>(a, s, grid_pct, dry);
+          case 11: return launch_shape<T, D, P, LO, CS, ES, 11>(a, s, grid_pct, dry);
           default: break;
         }
     }

@@ -164,8 +164,8 @@ VARIANT_DEFAULT, VARIANT_LDS, VARIANT_MFMA = 0, 1, 2
 # MFMA stage-kernel tile shapes (dgk_set_tile_shape): built-in, 4 waves x 16 elements, 8 waves x 32
 # elements, row split (4 waves x 4 elements), 4 waves x 16 elements with the operator in global memory,
 # row split over 8 waves x 8 elements, 4 waves x 16 elements compiled for 3, 4 or 5 waves per SIMD,
-# the last two with the epilogue operands loaded late
-TILE_SHAPES = (0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10)
+# the same for 4, 5 and 3 waves per SIMD with the epilogue operands loaded late
+TILE_SHAPES = (0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11)
 # fp32_flags: 0 = default FP32 form (compensated solution); PLAIN_STATE = plain FP32 solution;
 # OPERATOR_CORRECTION = add the bf16 operator-residual MFMA
 FP32_PLAIN_STATE, FP32_OPERATOR_CORRECTION = 1, 2

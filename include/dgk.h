@@ -136,8 +136,8 @@ dgk_status dgk_set_launch(dgk_ctx* ctx, int grid_pct, int tile_order);
  * the operator read from global memory; 5 the row split over 8-wave workgroups and 8-element tiles
  * (two element groups, two waves per SIMD); 6, 7, 8 4-wave workgroups over 16-element tiles compiled
  * for 3, 4 or 5 waves per SIMD (at most 512/n registers per lane; where that many workgroups' LDS
- * fits a CU); 9, 10 as 7, 8 with the RK epilogue operands loaded after the volume k-steps. Shapes
- * 1-10 exist for FP64 and the default FP32 form in
+ * fits a CU); 9, 10, 11 as 7, 8, 6 with the RK epilogue operands loaded after the volume k-steps.
+ * Shapes 1-11 exist for FP64 and the default FP32 form in
  * 2D and 3D at orders 2-5, for buffers below 4 GiB; a shape that has no kernel for the context, or
  * does not fit its element type, returns DGK_ERR_UNSUPPORTED and leaves the setting unchanged. If
  * the context's form later changes to one without that shape (a 2D state with nonzero vz), the

@@ -62,11 +62,11 @@ def _row_split(shape, dim, p, prec):
 
 
 def _waves_per_simd_budget(shape, dim, p, prec):
-    """Waves per SIMD the tile shape's registers are sized for (shapes 6-8: 3-5, 9-10: 4-5; the
-    built-in shape of FP32 3D order 4 is shape 6)."""
+    """Waves per SIMD the tile shape's registers are sized for (shapes 6-8: 3-5, 9-10: 4-5, 11: 3;
+    the built-in shape of FP32 3D order 4 is shape 6)."""
     if shape == 0:
         return 3 if (prec == dgk.FP32 and dim == 3 and p == 4) else 1
-    return shape - 3 if shape in (6, 7, 8) else shape - 5 if shape in (9, 10) else 1
+    return shape - 3 if shape in (6, 7, 8) else shape - 5 if shape in (9, 10) else 3 if shape == 11 else 1
 
 
 def _run(m, u, prec, shape=0, steps=30):
