@@ -66,9 +66,11 @@ def main():
     ap.add_argument("--receivers", type=int, default=0)
     ap.add_argument("--nsteps", type=int, default=2048)
     ap.add_argument("--stride", type=int, default=0, help="keep every stride-th sample")
-    ap.add_argument("--steps", type=int, default=0, help="training steps per network")
+    ap.add_argument("--steps", default="0", help="training steps per network (a comma-separated list: each in turn)")
     ap.add_argument("--lr", type=float, default=0.0, help="learning rate (default 2e-3 small, 1e-3 c5)")
-    ap.add_argument("--freq-sigma", type=float, default=0.1)
+    ap.add_argument("--freq-sigma", default="0.1",
+                    help="scale of the Fourier feature frequencies (per unit of the room-normalised "
+                         "coordinates); a comma-separated list trains and reports each in turn on the same data")
     ap.add_argument("--tol", type=float, default=0.02)
     ap.add_argument("--out", default="")
     ap.add_argument("--dump", default="", help="write sources, receivers and DG series (float32) to this .npz")
@@ -80,7 +82,8 @@ def main():
     S = args.sources or (64 if small else 40)
     R = args.receivers or (256 if small else 128)
     stride = args.stride or (4 if small else 1)
-    steps = args.steps or (15000 if small else 6000)
+    step_list = [int(x) for x in str(args.steps).split(",")]
+    step_list = [x or (15000 if small else 6000) for x in step_list]
     lr = args.lr or (2e-3 if small else 1e-3)
     t0 = time.perf_counter()
     if small:
@@ -133,57 +136,62 @@ def main():
     base = None
     if args.base == "image":
         base = lambda s_, r_: surrogate.image_source_series(s_, r_, L, sigma, ph["c0"], times)  # noqa: E731
-    sur = surrogate.Surrogate(np.zeros(3), np.array(L), out_dim=args.nsteps // stride, scale=float(np.abs(T).max()),
-                              freq_sigma=args.freq_sigma, base=base)
-    t0 = time.perf_counter()
-    losses = sur.fit(X_src[tr], X_rcv[tr], T[tr], steps, batch=256, lr=lr)
-    t_train = time.perf_counter() - t0
-    # relative errors (held out, calibration, training) are taken over the pairs the pulse reaches
-    # within the window (series norm above 1e-3 of the largest); the silent rest have no relative
-    # error to speak of and are counted separately
-    loud = np.linalg.norm(T, axis=1) > 1e-3 * np.linalg.norm(T, axis=1).max()
-    audible = loud[te]
-    te, ca_loud, tr_loud = te[audible], ca[loud[ca]], tr[loud[tr]][:512]
-    cal = sur.calibrate(X_src[ca_loud], X_rcv[ca_loud], T[ca_loud], quantile=0.9)
-    y, est = sur.predict(X_src[te], X_rcv[te])
-    err = surrogate.rel_l2(y, T[te])
-    err_base = surrogate.rel_l2(base(X_src[te], X_rcv[te]), T[te]) if base is not None else None
-    ytr, _ = sur.predict(X_src[tr_loud], X_rcv[tr_loud])
-    err_tr = surrogate.rel_l2(ytr, T[tr_loud])
-    t_q = sur.nets[0].forward_ms(256, 20) * len(sur.nets) / 256 * 1e-3
-    acc = est <= args.tol
-    out = {
-        "room": {"name": args.room, "elements": m.K, "order": 3, "dofs": 4 * m.K * m.Np, "dt": dt,
-                 "pulse_sigma": sigma, "samples": args.nsteps // stride, "sample_stride": stride},
-        "data": {"sources": S, "receivers_per_source": R, "pairs_train": len(tr), "pairs_calibration": len(ca),
-                 "pairs_held_out": 512, "held_out_audible": int(audible.sum()),
-                 "calibration_audible": len(ca_loud), "dg_seconds": t_data,
-                 "batched_vs_single_rel_l2": consistency},
-        "network": {"params": sur.nets[0].nparam, "freq_sigma": args.freq_sigma, "train_steps": steps,
-                    "lr": lr, "train_seconds": t_train, "final_loss": [float(l[-50:].mean()) for l in losses]},
-        "held_out_rel_l2": {"median": float(np.median(err)), "mean": float(err.mean()),
-                            "p10": float(np.percentile(err, 10)), "frac_below_tol": float((err <= args.tol).mean())},
-        "base": args.base,
-        "held_out_rel_l2_base_alone": None if err_base is None else
-        {"median": float(np.median(err_base)), "p10": float(np.percentile(err_base, 10)),
-         "frac_below_tol": float((err_base <= args.tol).mean())},
-        "train_rel_l2_median": float(np.median(err_tr)),
-        "gate": {"tol": args.tol, "calibration_scale": cal, "accepted": float(acc.mean()),
-                 "accepted_err_median": float(np.median(err[acc])) if acc.any() else None,
-                 "accepted_within_tol": float((err[acc] <= args.tol).mean()) if acc.any() else None,
-                 "accepted_err_max": float(err[acc].max()) if acc.any() else None,
-                 "corr_estimate_error": float(np.corrcoef(est, err)[0, 1])},
-        "gate_by_tolerance": {f"{tol:g}": {"accepted": float((est <= tol).mean()),
-                                           "accepted_within_tol": float((err[est <= tol] <= tol).mean())
-                                           if (est <= tol).any() else None}
-                              for tol in (0.02, 0.03, 0.05, 0.1)},
-        "timing": {"query_us": t_q * 1e6, "dg_solve_s": t_dg, "speedup": t_dg / t_q},
-    }
-    line = json.dumps(out)
-    print(line)
     if args.out:
-        with open(args.out, "w") as f:
-            f.write(line + "\n")
+        open(args.out, "w").close()
+    te0 = te
+    for freq_sigma, steps in [(float(f), n) for f in str(args.freq_sigma).split(",") for n in step_list]:
+        te = te0
+        sur = surrogate.Surrogate(np.zeros(3), np.array(L), out_dim=args.nsteps // stride, scale=float(np.abs(T).max()),
+                                  freq_sigma=freq_sigma, base=base)
+        t0 = time.perf_counter()
+        losses = sur.fit(X_src[tr], X_rcv[tr], T[tr], steps, batch=256, lr=lr)
+        t_train = time.perf_counter() - t0
+        # relative errors (held out, calibration, training) are taken over the pairs the pulse reaches
+        # within the window (series norm above 1e-3 of the largest); the silent rest have no relative
+        # error to speak of and are counted separately
+        loud = np.linalg.norm(T, axis=1) > 1e-3 * np.linalg.norm(T, axis=1).max()
+        audible = loud[te]
+        te, ca_loud, tr_loud = te[audible], ca[loud[ca]], tr[loud[tr]][:512]
+        cal = sur.calibrate(X_src[ca_loud], X_rcv[ca_loud], T[ca_loud], quantile=0.9)
+        y, est = sur.predict(X_src[te], X_rcv[te])
+        err = surrogate.rel_l2(y, T[te])
+        err_base = surrogate.rel_l2(base(X_src[te], X_rcv[te]), T[te]) if base is not None else None
+        ytr, _ = sur.predict(X_src[tr_loud], X_rcv[tr_loud])
+        err_tr = surrogate.rel_l2(ytr, T[tr_loud])
+        t_q = sur.nets[0].forward_ms(256, 20) * len(sur.nets) / 256 * 1e-3
+        acc = est <= args.tol
+        out = {
+            "room": {"name": args.room, "elements": m.K, "order": 3, "dofs": 4 * m.K * m.Np, "dt": dt,
+                     "pulse_sigma": sigma, "samples": args.nsteps // stride, "sample_stride": stride},
+            "data": {"sources": S, "receivers_per_source": R, "pairs_train": len(tr), "pairs_calibration": len(ca),
+                     "pairs_held_out": 512, "held_out_audible": int(audible.sum()),
+                     "calibration_audible": len(ca_loud), "dg_seconds": t_data,
+                     "batched_vs_single_rel_l2": consistency},
+            "network": {"params": sur.nets[0].nparam, "freq_sigma": freq_sigma, "train_steps": steps,
+                        "lr": lr, "train_seconds": t_train, "final_loss": [float(l[-50:].mean()) for l in losses]},
+            "held_out_rel_l2": {"median": float(np.median(err)), "mean": float(err.mean()),
+                                "p10": float(np.percentile(err, 10)), "frac_below_tol": float((err <= args.tol).mean())},
+            "base": args.base,
+            "held_out_rel_l2_base_alone": None if err_base is None else
+            {"median": float(np.median(err_base)), "p10": float(np.percentile(err_base, 10)),
+             "frac_below_tol": float((err_base <= args.tol).mean())},
+            "train_rel_l2_median": float(np.median(err_tr)),
+            "gate": {"tol": args.tol, "calibration_scale": cal, "accepted": float(acc.mean()),
+                     "accepted_err_median": float(np.median(err[acc])) if acc.any() else None,
+                     "accepted_within_tol": float((err[acc] <= args.tol).mean()) if acc.any() else None,
+                     "accepted_err_max": float(err[acc].max()) if acc.any() else None,
+                     "corr_estimate_error": float(np.corrcoef(est, err)[0, 1])},
+            "gate_by_tolerance": {f"{tol:g}": {"accepted": float((est <= tol).mean()),
+                                               "accepted_within_tol": float((err[est <= tol] <= tol).mean())
+                                               if (est <= tol).any() else None}
+                                  for tol in (0.02, 0.03, 0.05, 0.1)},
+            "timing": {"query_us": t_q * 1e6, "dg_solve_s": t_dg, "speedup": t_dg / t_q},
+        }
+        line = json.dumps(out)
+        print(line, flush=True)
+        if args.out:
+            with open(args.out, "a") as f:
+                f.write(line + "\n")
 
 
 if __name__ == "__main__":
