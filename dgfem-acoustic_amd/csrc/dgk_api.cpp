@@ -858,7 +858,7 @@ dgk_status dgk_stage_addressing(int dim, int order, int precision, int64_t num_e
   if (!addr64 || dim < 1 || dim > 3 || order < 1 || order > 5 || num_elements <= 0 ||
       (precision != DGK_FP64 && precision != DGK_FP32))
     return fail(nullptr, DGK_ERR_INVALID, "bad addressing query");
-  if ((num_elements + kTilePad) * (dim + 1) >= ((int64_t)1 << 31))
+  if (num_elements >= ((int64_t)1 << 31) || (num_elements + kTilePad) * (dim + 1) >= ((int64_t)1 << 31))
     return fail(nullptr, DGK_ERR_UNSUPPORTED, "more than 2^31 element faces");
   int64_t np = 1;
   for (int k = 1; k <= dim; ++k) np = np * (order + k) / k;  // C(order + dim, dim)

@@ -144,6 +144,9 @@ def test_stage_addressing_switches_at_4_gib_per_array():
     with pytest.raises(dgk.DGKError) as e:
         dgk.stage_addressing(3, 1, dgk.FP64, (1 << 31) // 4)
     assert e.value.status == dgk.DGK_ERR_UNSUPPORTED
+    with pytest.raises(dgk.DGKError) as e:  # no overflow in the face count of absurd sizes
+        dgk.stage_addressing(1, 1, dgk.FP64, (1 << 62) + 1)
+    assert e.value.status == dgk.DGK_ERR_UNSUPPORTED
     with pytest.raises(dgk.DGKError) as e:
         dgk.stage_addressing(4, 1, dgk.FP64, 10)
     assert e.value.status == dgk.DGK_ERR_INVALID
