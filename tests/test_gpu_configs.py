@@ -9,6 +9,8 @@
   (14,376 tets, order 4), FP64 and FP32 against the oracle for 100 steps.
 * The 3D P3 cube at full size (196,608 tets, mean flow, absorbing walls) for 50 steps, FP64
   and FP32, against the oracle.
+* The whole 256-member ensemble at full size: each FP64 batch member against the same member run
+  alone, each FP32 batch member against the FP64 batch.
 
 FP64 bar 1e-12 relative L2 over the whole state (the drop-in contract); FP32 1e-5 per field.
 Per field the FP64 velocities of order-5 members reach about 1e-12 after 30 steps: that is the
@@ -80,6 +82,59 @@ def test_ensemble_generator_16_members():
     meshes, phys, dts, pulses = wl.ensemble(n=16, kmin=500, kmax=3000, threads=8)
     assert sum(m.dim == 2 for m in meshes) == 8
     run_batch_against_oracle(meshes, phys, dts, pulses, "ensemble16")
+
+
+@pytest.mark.timeout(1200)
+def test_ensemble256_full_size_batch_equals_members_alone():
+    """The benchmarked ensemble at full size (256 members, 11.1 M elements, 0.89 G unknowns), read
+    after 20 and after 200 steps. FP64: every member of the batch equals that member run alone as
+    a plain FP64 context (one reference `rungeKutta` run each, src/solver.cpp:171-292) to 1e-12
+    relative L2; the batch folds each member's dt into its geometry, the single context applies it
+    per stage, so the two agree to rounding, not bitwise. FP32: every field of every member of the
+    FP32 batch within 1e-5 relative L2 of the FP64 batch, the FP32 form's stated bound. The
+    single-context path itself is held to the oracle by the tests above."""
+    meshes, phys, dts, pulses = wl.ensemble(n=256, threads=8)
+    assert sum(m.K for m in meshes) > 11_000_000
+    batches = {}
+    for prec in (dgk.FP64, dgk.FP32):
+        b = dgk.Batch(meshes, phys, dts, precision=prec, flux_sign=dgk.SIGN_STABLE)
+        for i, (m, pu) in enumerate(zip(meshes, pulses)):
+            b.set_state(i, wl.ensemble_state(m, pu))
+        batches[prec] = b
+    names = ("p", "vx", "vy", "vz")
+    done = 0
+    try:
+        for steps in (20, 200):
+            print(f"ensemble256 full size: both batches to step {steps}", flush=True)
+            for b in batches.values():
+                b.rk4(steps - done)
+            done = steps
+            worst64, worst32 = 0.0, {}
+            for i, (m, ph, dt, pu) in enumerate(zip(meshes, phys, dts, pulses)):
+                s = dgk.Solver(m, ph["c0"], ph["rho0"], v0=ph["v0"], precision=dgk.FP64, flux_sign=dgk.SIGN_STABLE)
+                try:
+                    s.set_state(wl.ensemble_state(m, pu))
+                    s.rk4(steps, dt)
+                    alone = s.get_state()
+                finally:
+                    s.close()
+                g64, g32 = batches[dgk.FP64].get_state(i), batches[dgk.FP32].get_state(i)
+                e64 = rel(g64, alone)
+                e32 = [rel(g32[q], g64[q]) for q in range(m.dim + 1)]
+                worst64 = max(worst64, e64)
+                for q, e in enumerate(e32):
+                    if e > worst32.get(names[q], (0.0,))[0]:
+                        worst32[names[q]] = (e, i, m.dim, m.order, m.K)
+                print(f"  step {steps} member {i}: dim {m.dim} order {m.order} K {m.K}: FP64 batch vs alone "
+                      f"{e64:.2e}, FP32 vs FP64 batch per field {' '.join(f'{e:.2e}' for e in e32)}", flush=True)
+                assert e64 <= 1e-12, (steps, i, m.dim, m.order, m.K, e64)
+                assert max(e32) <= 1e-5, (steps, i, m.dim, m.order, m.K, e32)
+                assert not g64[m.dim + 1:].any() and not g32[m.dim + 1:].any()
+            print(f"ensemble256 full size, step {steps}: worst FP64 batch vs alone {worst64:.2e}; worst FP32 "
+                  f"per field (rel L2, member, dim, order, K) {worst32}", flush=True)
+    finally:
+        for b in batches.values():
+            b.close()
 
 
 def room_receivers(m, center, seed=6):
