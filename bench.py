@@ -20,7 +20,11 @@ are (dim+1) fields per node. Synthetic data: meshes and states are generated her
 * c1_fp64 -- the reference's CPU-runnable case (2D unit square, 3200 triangles, order 3, 1000
   steps, FP64, reference flux sign), with the oracle's literal path timed on the host beside it.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--legs fp64,fp32,batch,c1]
+* host_inclusive (optional leg `hostio`) -- the FP64 cube and the FP32 ensemble with the state
+  handed in and out through host memory, as the C ABI's state calls take it, beside the same steps
+  resident in HBM.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--legs fp64,fp32,batch,c1,hostio]
 """
 import argparse
 import json
@@ -387,13 +391,81 @@ def cube_cpu(steps=8, n=12):
             "factorized_value": w / tf}
 
 
+def host_io_leg(steps):
+    """Rates with the state handed over in host memory, as the C ABI takes and returns it
+    (`dgk_set_state` / `dgk_get_state` and their batch forms: FP64 host arrays, 4 field planes):
+    host state in, `steps` RK4 steps, host state out, each phase timed on its own after a warm
+    run, beside the same steps with the state resident in HBM (`value`'s form). The FP64 cube (the
+    drop-in form) and the FP32 ensemble."""
+    out = {}
+    m, ph, dt, u0 = wl.cube_3d()
+    dof = 4 * m.K * m.Np
+    s = dgk.Solver(m, ph["c0"], ph["rho0"], v0=ph["v0"], precision=dgk.FP64, flux_sign=dgk.SIGN_STABLE)
+    s.set_state(u0)
+    s.rk4(min(steps, 25), dt)
+    s.get_state()
+    t = {}
+    s.sync()
+    t0 = time.perf_counter()
+    s.set_state(u0)
+    s.sync()
+    t["in"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    s.rk4(steps, dt)
+    s.sync()
+    t["run"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    s.get_state()
+    t["out"] = time.perf_counter() - t0
+    s.close()
+    out["cube_fp64"] = {"steps": steps, "host_bytes_each_way": int(u0.nbytes), "host_in_s": t["in"],
+                        "resident_s": t["run"], "host_out_s": t["out"], "resident_rate": dof * 4 * steps / t["run"],
+                        "host_inclusive_rate": dof * 4 * steps / (t["in"] + t["run"] + t["out"])}
+    log(f"hostio cube fp64: {out['cube_fp64']['host_inclusive_rate']:.4g} with host state in and out, "
+        f"{out['cube_fp64']['resident_rate']:.4g} resident")
+    meshes, phys, dts, pulses = wl.ensemble(n=256)
+    states = [wl.ensemble_state(mm, pu) for mm, pu in zip(meshes, pulses)]
+    dof = wl.dof_count(meshes)
+    b = dgk.Batch(meshes, phys, dts, precision=dgk.FP32, flux_sign=dgk.SIGN_STABLE)
+    for i, u in enumerate(states):
+        b.set_state(i, u)
+    schedule, _ = b.pick_schedule(10)
+    b.rk4(min(steps, 100), schedule)
+    for i in range(len(states)):
+        b.get_state(i)
+    b.sync()
+    t0 = time.perf_counter()
+    for i, u in enumerate(states):
+        b.set_state(i, u)
+    b.sync()
+    t["in"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.rk4(steps, schedule)
+    b.sync()
+    t["run"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for i in range(len(states)):
+        b.get_state(i)
+    t["out"] = time.perf_counter() - t0
+    b.close()
+    out["ensemble_fp32"] = {"steps": steps, "host_bytes_each_way": int(sum(u.nbytes for u in states)),
+                            "host_in_s": t["in"], "resident_s": t["run"], "host_out_s": t["out"],
+                            "resident_rate": dof * 4 * steps / t["run"],
+                            "host_inclusive_rate": dof * 4 * steps / (t["in"] + t["run"] + t["out"])}
+    log(f"hostio ensemble fp32: {out['ensemble_fp32']['host_inclusive_rate']:.4g} with host state in and out, "
+        f"{out['ensemble_fp32']['resident_rate']:.4g} resident")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="fp64,fp32,batch,c1", help="comma list of fp64,fp32,batch,c1")
+    ap.add_argument("--legs", default="fp64,fp32,batch,c1",
+                    help="comma list of fp64,fp32,batch,c1,hostio (hostio: rates with the state handed over "
+                         "in host memory; not in the default run)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="first run a plain device copy of MB MiB (reference dispatch for HBM byte counters)")
     args = ap.parse_args()
@@ -473,6 +545,8 @@ def main():
             out["fp32_single"]["rel_l2_vs_fp64"] = {"after_steps": r["total_steps"], **r["errs"]}
     if "c1" in res:
         out["c1_fp64"] = res["c1"]
+    if "hostio" in legs:
+        out["host_inclusive"] = host_io_leg(args.steps)
     out["cpu_baseline"] = None
     if not args.no_cpu and "batch" in res:
         log("cpu baseline: literal oracle on a sample of the ensemble, all host threads")
