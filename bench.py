@@ -192,11 +192,17 @@ def batch_leg(steps, warmup, members_checked=8, n=256):
     groups = b.groups()
     log(f"batch: device setup and state upload {time.perf_counter() - t0:.1f} s, {len(groups)} groups")
     b.rk4(warmup)
-    # schedule: groups as parallel graph branches or in sequence, whichever runs a short
-    # captured trial faster on this GPU (10 steps each)
+    # tile shape and launch shape of every group at its own size, then the schedule: groups as
+    # parallel graph branches or in sequence, whichever runs a short captured trial faster on this
+    # GPU (10 steps each)
+    t0 = time.perf_counter()
+    tuned, tune_steps = b.tune_groups()
+    log(f"batch: group tuning ({tune_steps} event-timed steps, {time.perf_counter() - t0:.1f} s): " +
+        ", ".join(f"{g['dim']}D p{g['order']} shape {c[0]} grid {c[1]} order {c[2]} ({c[3]:.1f} us)"
+                  for g, c in zip(groups, tuned)))
     schedule, trial = b.pick_schedule(10)
     log(f"batch: schedule trial (10 steps) parallel {trial[0]:.3f} s, sequential {trial[1]:.3f} s")
-    total_before = warmup + 20
+    total_before = warmup + tune_steps + 20
     t0 = time.perf_counter()
     b.prepare(steps, schedule)
     log(f"batch: captured {steps} steps x {len(groups)} groups in {time.perf_counter() - t0:.1f} s")
@@ -210,11 +216,12 @@ def batch_leg(steps, warmup, members_checked=8, n=256):
     gms = b.rk4_timed(nk)
     total = total_before + steps + nk
     gtab = []
-    for g, ms in zip(groups, gms):
+    for g, ms, c in zip(groups, gms, tuned):
         Np = element_sizes(g["dim"], g["order"])[0]
         bl = algorithmic_bytes(g["dim"], g["order"], g["elements"], g["faces"], g["boundary_faces"], 4, True)
         st = ms * 1e-3 / (4 * nk)
         gtab.append(dict(dim=g["dim"], order=g["order"], sims=g["sims"], elements=g["elements"],
+                         tile_shape=c[0], grid_pct=c[1], tile_order=c[2],
                          dof=(g["dim"] + 1) * g["elements"] * Np, stage_us=round(st * 1e6, 1), bytes=bl,
                          flops=algorithmic_flops(g["dim"], g["order"], g["elements"]), stage_s=st))
     tot = sum(x["stage_s"] for x in gtab)

@@ -494,6 +494,46 @@ class Batch:
             SCHEDULE_SEQUENTIAL
         return best, trial
 
+    def tune_groups(self, steps=3, grids=(100, 200, 400), orders=(0, 1), margin=0.98):
+        """Tile-shape and launch-shape autotuning of every group at its own size on this GPU (the
+        tuning table is measured on one mesh per (dim, order, precision); a group's size and mix of
+        members are its own). Each candidate (tile shape, grid_pct, tile_order) is set on every
+        group that has a kernel of that shape, and all groups are timed at once with event-timed
+        stage launches (`steps` RK4 steps per candidate, which the members advance); each group
+        keeps its fastest candidate, one of another tile shape than the built-in one only when it
+        beats the built-in shape's best by `margin`. Results do not depend on the launch shape and
+        agree with the built-in tile shape to rounding. Returns (choices, steps advanced): per
+        group, (tile_shape, grid_pct, tile_order, stage_us) of the choice, which stays set."""
+        n = len(self.groups())
+        times = [dict() for _ in range(n)]
+        advanced = 0
+        for sh in TILE_SHAPES:
+            ok = []
+            for gi in range(n):
+                try:
+                    self.set_tile_shape(gi, sh)
+                    ok.append(gi)
+                except DGKError as e:
+                    if e.status != DGK_ERR_UNSUPPORTED:
+                        raise
+            for g in grids if ok else ():
+                for o in orders:
+                    for gi in ok:
+                        self.set_launch(gi, g, o)
+                    ms = self.rk4_timed(steps)
+                    advanced += steps
+                    for gi in ok:
+                        times[gi][(sh, g, o)] = ms[gi] * 1e3 / (4 * steps)
+        out = []
+        for gi in range(n):
+            base = min((t, k) for k, t in times[gi].items() if k[0] == 0)
+            best = min((t, k) for k, t in times[gi].items())
+            t, (sh, g, o) = best if best[0] < margin * base[0] else base
+            self.set_tile_shape(gi, sh)
+            self.set_launch(gi, g, o)
+            out.append((sh, g, o, t))
+        return out, advanced
+
     def rk4_timed(self, nsteps):
         """Run nsteps per group with event-timed stage launches; summed stage ms per group."""
         n = len(self.groups())

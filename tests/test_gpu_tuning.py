@@ -169,3 +169,47 @@ def test_tile_shape_fallback_and_batch_groups(tmp_path):
         assert L.dgk_load_tuning((dgk.TUNING_FILE if os.path.exists(dgk.TUNING_FILE) else str(empty)).encode()) == 0
     assert np.array_equal(t3, row3)
     assert np.array_equal(t2, ref2)
+
+
+def test_batch_group_tuning_keeps_results():
+    """Batch.tune_groups times every (tile shape, grid, tile order) candidate on every group with a
+    kernel of that shape and keeps each group's fastest: the choices are valid shapes of the groups,
+    the members advance exactly the timed steps, and a run after tuning agrees with an untuned batch
+    advanced by the same steps to FP32 rounding."""
+    m2, m3 = mixed_mesh(2, 3, 9), mixed_mesh(3, 4, 9)
+    u2 = np.random.default_rng(8).standard_normal((4, m2.K * m2.Np))
+    u3 = np.random.default_rng(9).standard_normal((4, m3.K * m3.Np))
+    u2[3] = 0
+    out = []
+    for tune in (False, True):
+        b = dgk.Batch([m2, m3], [dict(c0=340.0, rho0=1.225, v0=(30.0, -10.0, 5.0))] * 2, [1e-6, 1e-6],
+                      precision=dgk.FP32, flux_sign=dgk.SIGN_STABLE)
+        try:
+            b.set_state(0, u2)
+            b.set_state(1, u3)
+            if tune:
+                choices, advanced = b.tune_groups(steps=1, grids=(100, 200), orders=(0, 1))
+                assert len(choices) == 2
+                for sh, g, o, us in choices:
+                    assert sh in dgk.TILE_SHAPES and g in (100, 200) and o in (0, 1) and us > 0
+                # 2 launch grids x 2 tile orders per tile shape that some group has a kernel of
+                assert advanced % 4 == 0 and advanced >= 4 * 8, advanced
+            else:
+                advanced = None
+            out.append((b, advanced))
+        except BaseException:
+            b.close()
+            raise
+    (b0, _), (b1, advanced) = out
+    try:
+        b0.rk4(advanced)
+        b0.rk4(20)
+        b1.rk4(20)
+        for i in (0, 1):
+            x, y = b1.get_state(i), b0.get_state(i)
+            err = np.linalg.norm(x - y) / np.linalg.norm(y)
+            print(f"  member {i}: tuned vs untuned rel L2 {err:.2e}", flush=True)
+            assert err <= 1e-5
+    finally:
+        b0.close()
+        b1.close()
