@@ -240,3 +240,43 @@ def test_interpolation_helper_reproduces_polynomials(dim, p):
     ext = np.array([1.0, 0.8 if dim > 1 else 0.0, 0.7 if dim > 2 else 0.0])
     pts = 0.05 * ext + 0.9 * ext * rng.random((6, 3))
     assert np.allclose(sample(m, u, pts), f(pts), rtol=1e-10, atol=1e-10)
+
+
+def test_batch_group_tuning_choice_rule():
+    """Batch.tune_groups' host logic, with the device calls stubbed: every candidate is set on the
+    groups that have a kernel of that shape (others refuse with DGK_ERR_UNSUPPORTED and are not
+    timed on it), each group keeps its fastest candidate, one of another tile shape than the
+    built-in one only when it is below 0.98 of the built-in shape's best, and the choice is left
+    set on the group."""
+    # stage us per (group, shape, grid, order); group 1 has no kernel of shape 3
+    table = {0: {0: 100.0, 1: 97.0, 3: 99.5}, 1: {0: 50.0, 1: 49.5, 2: 40.0}}
+    grid_gain = {100: 0.0, 200: -1.0}
+    b = object.__new__(dgk.Batch)
+    state = {0: [0, 100, 0], 1: [0, 100, 0]}
+    calls = []
+
+    def set_tile_shape(gi, sh):
+        if sh not in table[gi]:
+            raise dgk.DGKError(dgk.DGK_ERR_UNSUPPORTED, "no kernel")
+        state[gi][0] = sh
+
+    def set_launch(gi, g, o):
+        state[gi][1:] = [g, o]
+
+    def rk4_timed(steps):
+        calls.append(steps)
+        return np.array([(table[gi][state[gi][0]] + grid_gain[state[gi][1]]) * 4 * steps / 1e3 for gi in (0, 1)])
+
+    b.groups = lambda: [{}, {}]
+    b.set_tile_shape, b.set_launch, b.rk4_timed = set_tile_shape, set_launch, rk4_timed
+    choices, advanced = b.tune_groups(steps=2, grids=(100, 200), orders=(0, 1), margin=0.98)
+    # group 0: shape 1 at 96 us beats 0.98 x 99 (the built-in best); shape 3 at 98.5 does not
+    # group 1: shape 2 at 39 us beats 0.98 x 49
+    assert [c[:3] for c in choices] == [(1, 200, 0), (2, 200, 0)]
+    assert choices[0][3] == pytest.approx(96.0) and choices[1][3] == pytest.approx(39.0)
+    assert state == {0: [1, 200, 0], 1: [2, 200, 0]}
+    # 4 tile shapes with a kernel on some group x 2 grids x 2 tile orders, 2 steps each
+    assert advanced == len(calls) * 2 == 4 * 4 * 2
+    table[0][1] = 98.5  # 97.5 with the grid gain, above 0.98 x 99 = 97.02: the built-in shape stays
+    choices, _ = b.tune_groups(steps=1, grids=(100, 200), orders=(0,), margin=0.98)
+    assert choices[0][:3] == (0, 200, 0)
