@@ -651,9 +651,14 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P, SH>::KB), (MfmaLaunch<T, D, P,
     // absorbing   qp = (0, v - (v.n) n),   A = 1/4, vne = 0,    H = c0/4
     //             (= 1/4 [c0 p + rc2 v.n ; (p/rho + c0 v.n) n]; the qp part of F vanishes up to the
     //             rounding of v.n - (v.n)(n.n))
+    // register budgets of three or more waves per SIMD: the face nodes' LDS addresses are
+    // recomputed each tile (a few integer operations) rather than held across the tile loop, where
+    // the compiler spilled them to scratch and reloaded them every tile
+    int ftid = tid;
+    if constexpr (MfmaLaunch<T, D, P, SH>::MINW >= 3) asm volatile("" : "+v"(ftid));
 #pragma unroll
     for (int t = 0; t < NI; ++t) {
-      const int idx = tid + t * kBlock;
+      const int idx = ftid + t * kBlock;
       if ((t + 1) * kBlock > TE * NFF && idx >= TE * NFF) continue;  // only the last round is partial
       const int e = idx / NFF, jk = idx - e * NFF, j = jk / NFP;
       const int ej = e * NF + j;
