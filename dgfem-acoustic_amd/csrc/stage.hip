@@ -651,11 +651,13 @@ __global__ __launch_bounds__((MfmaLaunch<T, D, P, SH>::KB), (MfmaLaunch<T, D, P,
     // absorbing   qp = (0, v - (v.n) n),   A = 1/4, vne = 0,    H = c0/4
     //             (= 1/4 [c0 p + rc2 v.n ; (p/rho + c0 v.n) n]; the qp part of F vanishes up to the
     //             rounding of v.n - (v.n)(n.n))
-    // register budgets of three or more waves per SIMD: the face nodes' LDS addresses are
-    // recomputed each tile (a few integer operations) rather than held across the tile loop, where
-    // the compiler spilled them to scratch and reloaded them every tile
+    // the face nodes' LDS addresses are recomputed each tile (a few integer operations) rather than
+    // held across the tile loop: under register budgets of three or more waves per SIMD the
+    // compiler spilled them to scratch and reloaded them every tile, and in 1D/2D the registers
+    // saved raise the waves per SIMD (2D order 2 FP32: 5 -> 6, 10% faster); the 3D kernels without
+    // a budget keep them (the extra integer work slows the issue-bound order-5 kernel by 1%)
     int ftid = tid;
-    if constexpr (MfmaLaunch<T, D, P, SH>::MINW >= 3) asm volatile("" : "+v"(ftid));
+    if constexpr (MfmaLaunch<T, D, P, SH>::MINW >= 3 || D < 3) asm volatile("" : "+v"(ftid));
 #pragma unroll
     for (int t = 0; t < NI; ++t) {
       const int idx = ftid + t * kBlock;
